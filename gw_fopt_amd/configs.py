@@ -1,0 +1,119 @@
+"""The workloads of the bulk-flow path (seeded, reproducible).
+
+Each returns a ``Workload``: interior bubbles, their six periodic face
+images, angular grid, time grid, omegas and k-hat list.
+
+  plain_8       8 bubbles at t = 0 in a periodic unit cube; 32 x 64 surface
+                grid; 200 time points on [0, 1.5]; 64 omegas geomspace(0.5,
+                200); 16 Fibonacci k-hat; scaling f = 1 * ratio^5 (xi = 2).
+  plain_256     256 bubbles from exponential nucleation beta L = 20 (first 256
+                nuclei, rejection inside existing bubbles); 64 x 128; 1000 time
+                points on [0, t_percolation]; 256 omegas geomspace(0.5, 500);
+                64 Fibonacci k-hat; xi = 2.
+  mixed_8       plain_8 in the mixed-precision form.
+  ensemble_512  512 realisations, beta L ~ U[10, 30], box sides giving ~20-400
+                bubbles (log-uniform targets, mean ~128); 32 x 64; 500 time
+                points to each realisation's percolation; 128 omegas
+                geomspace(0.5, 300) / L; 32 k-hat.
+  grad_128      128 bubbles, beta L = 15; 32 x 64; 500 time points; 128
+                omegas geomspace(0.5, 300); 32 k-hat; theta = (xi, A) with
+                coefficients [[A]], powers [[xi + 3]]; target theta* = (2.2, 0.9).
+
+The scaling function follows the reference's convention f = sum_k a_k
+ratio^{p_k} with p = xi + 3 (src/gw_fopt/bubble_dynamics/utils/
+data_gw_bulkflow.py:86, docs/generalized_bulkflow.md:45-46).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import nucleation
+from .many_bubbles import fibonacci_directions
+
+
+@dataclass
+class Workload:
+    name: str
+    interior: np.ndarray
+    exterior: np.ndarray
+    n_cos_thetax: int
+    n_phix: int
+    t_begin: float
+    t_end: float
+    n_t: int
+    w_arr: np.ndarray
+    k_hat: np.ndarray
+    coefficients_sets: list = field(default_factory=lambda: [[1.0]])
+    powers_sets: list = field(default_factory=lambda: [[5.0]])
+    box: float = 1.0
+
+    @property
+    def n_bubbles(self) -> int:
+        return len(self.interior)
+
+    def points(self) -> int:
+        return len(self.w_arr) * len(self.k_hat)
+
+
+def theta_to_params(xi: float, amplitude: float):
+    """(xi, A) -> (coefficients_sets, powers_sets) = ([[A]], [[xi + 3]])."""
+    return [[float(amplitude)]], [[float(xi) + 3.0]]
+
+
+def plain_8(seed: int = 1) -> Workload:
+    interior = nucleation.simultaneous_nucleation(8, box=1.0, seed=seed)
+    return Workload("plain_8", interior, nucleation.face_images(interior, 1.0), 32, 64, 0.0, 1.5,
+                    200, np.geomspace(0.5, 200.0, 64), fibonacci_directions(16))
+
+
+def plain_256(seed: int = 2, n_k: int = 64) -> Workload:
+    interior = nucleation.exponential_nucleation(20.0, n_max=256, seed=seed)
+    t_end = nucleation.percolation_time(interior)
+    return Workload("plain_256", interior, nucleation.face_images(interior, 1.0), 64, 128, 0.0,
+                    t_end, 1000, np.geomspace(0.5, 500.0, 256), fibonacci_directions(n_k))
+
+
+def grad_128(seed: int = 5, n_k: int = 32) -> Workload:
+    interior = nucleation.exponential_nucleation(15.0, n_max=128, seed=seed)
+    t_end = nucleation.percolation_time(interior)
+    c, p = theta_to_params(2.2, 0.9)
+    return Workload("grad_128", interior, nucleation.face_images(interior, 1.0), 32, 64, 0.0,
+                    t_end, 500, np.geomspace(0.5, 300.0, 128), fibonacci_directions(n_k), c, p)
+
+
+@dataclass
+class Ensemble:
+    name: str
+    interiors: list
+    exteriors: list
+    boxes: np.ndarray
+    t_end: np.ndarray
+    n_cos_thetax: int
+    n_phix: int
+    n_t: int
+    x_arr: np.ndarray          # omega * L (shared); realisation r uses x_arr / L_r
+    k_hat: np.ndarray
+    coefficients_sets: list = field(default_factory=lambda: [[1.0]])
+    powers_sets: list = field(default_factory=lambda: [[5.0]])
+
+    def w_arr(self) -> np.ndarray:
+        return self.x_arr[None, :] / self.boxes[:, None]
+
+
+def ensemble_512(n_real: int = 512, seed: int = 4, n_k: int = 32) -> Ensemble:
+    rng = np.random.default_rng(seed)
+    interiors, exteriors, boxes, t_end = [], [], [], []
+    for r in range(n_real):
+        beta = rng.uniform(10.0, 30.0)
+        n_target = float(np.exp(rng.uniform(np.log(20.0), np.log(400.0))))
+        # bubbles at percolation ~ 0.7 V beta^3 / (8 pi): box side for the target count
+        box = float((n_target * 8.0 * np.pi / (1.47 * beta ** 3)) ** (1.0 / 3.0))
+        b = nucleation.exponential_nucleation(beta, box=box, n_max=400, seed=seed * 100003 + r)
+        interiors.append(b)
+        exteriors.append(nucleation.face_images(b, box))
+        boxes.append(box)
+        t_end.append(nucleation.percolation_time(b, box=box, n_probe=8192))
+    return Ensemble("ensemble_512", interiors, exteriors, np.array(boxes), np.array(t_end), 32, 64,
+                    500, np.geomspace(0.5, 300.0, 128), fibonacci_directions(n_k))

@@ -1,0 +1,1258 @@
+// SPDX-License-Identifier: MIT
+// Host side of libbubble_gw_c.so: the reference's C ABI
+// (rust/c-bubble-gw/src/lib.rs) over the gfx950 kernels.
+//
+// Host work is limited to what the reference also does on the host once per
+// configuration: copying and validating the bubbles (LatticeBubbles::new,
+// rust/bubble-gw/src/many_bubbles/lattice_bubbles.rs:211-308), building the
+// angular grids (generalized_bulk_flow.rs:208-270), and sorting each
+// reference bubble's partners by a collision-delay lower bound.  Everything
+// per k-hat and per time step runs on the device.  There is no CPU fallback:
+// without a HIP device every compute call fails with CalculationError.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "bgw_common.h"
+#include "bgw_kernels.h"
+#include "bubble_gw.h"
+
+using bgw::CandEntry;
+using bgw::Item;
+using bgw::RealInfo;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;  // std::f64::consts::PI
+constexpr size_t kMaxSetsPerLaunch = 6;
+
+// ---------------------------------------------------------------------------
+// Errors (lib.rs:14-16, 104-134)
+// ---------------------------------------------------------------------------
+thread_local std::string t_last_error;
+
+struct CalcError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void set_last_error(std::string message) {
+  for (char& c : message)
+    if (c == '\0') c = ' ';
+  t_last_error = std::move(message);
+}
+
+template <class F>
+BgwStatus run_ffi(F&& f) {
+  try {
+    f();
+    t_last_error.clear();
+    return BgwStatus_Ok;
+  } catch (const CalcError& e) {
+    set_last_error(e.what());
+    return BgwStatus_CalculationError;
+  } catch (const std::exception& e) {
+    set_last_error(std::string("C++ exception crossed the C ABI boundary: ") + e.what());
+    return BgwStatus_Panic;
+  } catch (...) {
+    set_last_error("C++ exception crossed the C ABI boundary");
+    return BgwStatus_Panic;
+  }
+}
+
+std::string null_error(const char* name) { return std::string(name) + " must not be null"; }
+
+template <class T>
+const T* slice_from_raw(const T* p, size_t len, const char* name) {
+  if (len == 0) return p;
+  if (!p) throw CalcError(null_error(name));
+  return p;
+}
+
+size_t checked_product(size_t a, size_t b, const char* name) {
+  if (a != 0 && b > std::numeric_limits<size_t>::max() / a)
+    throw CalcError(std::string(name) + " length overflow");
+  return a * b;
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw CalcError(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) hip_check((x), #x)
+
+// ---------------------------------------------------------------------------
+// Arithmetic shared with the kernels (compiled with -ffp-contract=off)
+// ---------------------------------------------------------------------------
+inline double lscalar(const double* u, const double* v) {
+  double a = (-u[0]) * v[0];
+  double b = u[1] * v[1];
+  double c = u[2] * v[2];
+  double d = u[3] * v[3];
+  a = a + c;
+  b = b + d;
+  return a + b;
+}
+
+inline double linspace_at(double a, double b, size_t n, size_t i) {
+  double step = n > 1 ? (b - a) / static_cast<double>(n - 1) : 0.0;
+  return a + step * static_cast<double>(i);
+}
+
+// Rotation taking the unit vector k to z-hat: R = R_y(-theta) R_z(-phi); its
+// last row is k itself, so the rotated z of a point is k . x.
+std::array<double, 9> rotation_to_z(const double* k) {
+  double st = std::sqrt(k[0] * k[0] + k[1] * k[1]);
+  double cp = 1.0, sp = 0.0;
+  if (st > 0.0) {
+    cp = k[0] / st;
+    sp = k[1] / st;
+  }
+  double ct = k[2];
+  return {ct * cp, ct * sp, -st, -sp, cp, 0.0, k[0], k[1], k[2]};
+}
+
+std::string bubble_index_name(size_t b, size_t n_int) {
+  return b < n_int ? "Interior(" + std::to_string(b) + ")"
+                   : "Exterior(" + std::to_string(b - n_int) + ")";
+}
+
+// ---------------------------------------------------------------------------
+// Bubbles of one realisation and their sorted candidate lists
+// ---------------------------------------------------------------------------
+struct Bubbles {
+  size_t n_int = 0, n_ext = 0;
+  std::vector<double> xyz;  // (n_int + n_ext, 4): interior rows then exterior rows
+  bool nonfinite = false;
+  size_t n_tot() const { return n_int + n_ext; }
+  const double* row(size_t b) const { return &xyz[b * 4]; }
+};
+
+Bubbles make_bubbles(const double* interior, size_t n_int, const double* exterior, size_t n_ext) {
+  Bubbles B;
+  B.n_int = n_int;
+  B.n_ext = n_ext;
+  B.xyz.resize((n_int + n_ext) * 4);
+  if (n_int) std::memcpy(B.xyz.data(), interior, n_int * 4 * sizeof(double));
+  if (n_ext) std::memcpy(B.xyz.data() + n_int * 4, exterior, n_ext * 4 * sizeof(double));
+  for (double v : B.xyz)
+    if (!std::isfinite(v)) B.nonfinite = true;
+  // check_bubble_formed_inside_bubble (lattice_bubbles.rs:71-119), in the
+  // reference's loop order: interior-interior, then interior-exterior.
+  for (size_t a = 0; a < n_int; ++a)
+    for (size_t b = a + 1; b < n_int; ++b) {
+      const double* xa = B.row(a);
+      const double* xb = B.row(b);
+      double d[4] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2], xb[3] - xa[3]};
+      if (lscalar(d, d) < 0.0)
+        throw CalcError("Bubble " + bubble_index_name(a, n_int) + " is formed inside bubble " +
+                        bubble_index_name(b, n_int) +
+                        " at initial time (overlapping light cones)");
+    }
+  for (size_t a = 0; a < n_int; ++a)
+    for (size_t e = 0; e < n_ext; ++e) {
+      const double* xa = B.row(a);
+      const double* xb = B.row(n_int + e);
+      double d[4] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2], xb[3] - xa[3]};
+      if (lscalar(d, d) < 0.0)
+        throw CalcError("Bubble " + bubble_index_name(a, n_int) + " is formed inside bubble " +
+                        bubble_index_name(n_int + e, n_int) +
+                        " at initial time (overlapping light cones)");
+    }
+  return B;
+}
+
+// Candidate list of reference bubble a: every other bubble b with
+// delta = x_b - x_a (bitwise what LatticeBubbles::new stores, :280-297), its
+// squared interval, and the delay lower bound (|delta_s| + delta_t) / 2 that
+// holds along every direction for pairs well outside the light cone.
+void build_candidates(const Bubbles& B, size_t a, CandEntry* out, int32_t* inv) {
+  const size_t n_tot = B.n_tot();
+  const double* xa = B.row(a);
+  size_t m = 0;
+  for (size_t b = 0; b < n_tot; ++b) {
+    if (b < B.n_int && b == a) continue;
+    const double* xb = B.row(b);
+    CandEntry e{};
+    for (int k = 0; k < 4; ++k) e.d[k] = xb[k] - xa[k];
+    e.dsq = lscalar(e.d, e.d);
+    const double ns = std::sqrt((e.d[1] * e.d[1] + e.d[2] * e.d[2]) + e.d[3] * e.d[3]);
+    e.scale = std::fabs(e.d[0]) + ns;
+    const bool finite = std::isfinite(e.d[0]) && std::isfinite(e.d[1]) &&
+                        std::isfinite(e.d[2]) && std::isfinite(e.d[3]) && std::isfinite(e.dsq);
+    const bool regular = finite && e.dsq >= 1e-6 * (e.d[0] * e.d[0] + ns * ns) && e.dsq > 0.0;
+    e.key = regular ? 0.5 * (ns + e.d[0]) : -std::numeric_limits<double>::infinity();
+    e.idx = static_cast<int32_t>(b);
+    e.flags = regular ? bgw::kCandRegular : 0;
+    out[m++] = e;
+  }
+  std::sort(out, out + m, [](const CandEntry& x, const CandEntry& y) {
+    if (x.key != y.key) return x.key < y.key;
+    return x.idx < y.idx;
+  });
+  for (size_t b = 0; b < n_tot; ++b) inv[b] = -1;
+  for (size_t s = 0; s < m; ++s) inv[out[s].idx] = static_cast<int32_t>(s);
+}
+
+template <class F>
+void parallel_for(size_t n, size_t n_threads, F&& f) {
+  n_threads = std::max<size_t>(1, std::min(n_threads, n));
+  if (n_threads <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (size_t t = 0; t < n_threads; ++t)
+    pool.emplace_back([&] {
+      for (size_t i = next++; i < n; i = next++) f(i);
+    });
+  for (auto& th : pool) th.join();
+}
+
+// ---------------------------------------------------------------------------
+// Angular grids and scaling parameters
+// ---------------------------------------------------------------------------
+struct Resolution {
+  bool set = false;
+  size_t n_c = 0, n_phi = 0, n_phi_pow2 = 0;
+  std::vector<double> cos_t, sin_t, phi, cos_p, sin_p, gcos, gsin, ring_fac;
+};
+
+Resolution make_resolution(size_t n_c, size_t n_phi) {
+  if (n_c < 2 || n_phi < 2)
+    throw CalcError("Invalid resolution: Angular resolutions must be greater than one");
+  if (n_phi > 2048) throw CalcError("Invalid resolution: n_phix > 2048 is not supported");
+  if (n_c > 65535) throw CalcError("Invalid resolution: n_cos_thetax > 65535 is not supported");
+  Resolution R;
+  R.set = true;
+  R.n_c = n_c;
+  R.n_phi = n_phi;
+  R.n_phi_pow2 = 1;
+  while (R.n_phi_pow2 < n_phi) R.n_phi_pow2 <<= 1;
+  // set_resolution (generalized_bulk_flow.rs:221-238)
+  R.cos_t.resize(n_c);
+  R.sin_t.resize(n_c);
+  for (size_t i = 0; i < n_c; ++i) {
+    R.cos_t[i] = linspace_at(-1.0, 1.0, n_c, i);
+    R.sin_t[i] = std::fabs(std::sqrt(1.0 - R.cos_t[i] * R.cos_t[i]));
+  }
+  R.phi.resize(n_phi);
+  R.cos_p.resize(n_phi);
+  R.sin_p.resize(n_phi);
+  R.gcos.resize(n_phi);
+  R.gsin.resize(n_phi);
+  for (size_t j = 0; j < n_phi; ++j) {
+    R.phi[j] = linspace_at(0.0, 2.0 * kPi, n_phi, j);
+    R.sin_p[j] = std::sin(R.phi[j]);
+    R.cos_p[j] = std::cos(R.phi[j]);
+    // compute_b_integral (:528-539): trapezoid weights 1/2 at both ends
+    const double w = (j == 0 || j == n_phi - 1) ? 0.5 : 1.0;
+    R.gcos[j] = std::cos(2.0 * R.phi[j]) * w;
+    R.gsin[j] = std::sin(2.0 * R.phi[j]) * w;
+  }
+  // per-ring factor dcos * w_i * 0.5 * sin^2 * dphi (:522-523, 633-651)
+  const double dphi = 2.0 * kPi / static_cast<double>(n_phi);
+  const double dcos = 2.0 / static_cast<double>(n_c - 1);
+  R.ring_fac.resize(n_c);
+  for (size_t i = 0; i < n_c; ++i) {
+    const double sin_sq = 1.0 - R.cos_t[i] * R.cos_t[i];
+    const double wi = (i == 0 || i == n_c - 1) ? 0.5 : 1.0;
+    R.ring_fac[i] = dcos * wi * (0.5 * sin_sq) * dphi;
+  }
+  return R;
+}
+
+struct Scaling {
+  size_t n_sets = 1, n_terms = 1;
+  std::vector<double> coef{1.0}, pows{3.0};  // defaults (generalized_bulk_flow.rs:98-99)
+  bool has_damping = false;
+  double damping = 0.0;
+};
+
+Scaling make_scaling(const double* coefficients, const double* powers, size_t n_sets,
+                     size_t n_terms, double damping) {
+  const size_t len = checked_product(n_sets, n_terms, "gradient scaling");
+  slice_from_raw(coefficients, len, "coefficients");
+  slice_from_raw(powers, len, "powers");
+  if (n_sets > 0 && n_terms == 0)
+    throw CalcError("Array shape mismatch: n_terms must be > 0 when n_sets > 0");
+  Scaling S;
+  S.n_sets = n_sets;
+  S.n_terms = n_terms;
+  S.coef.assign(coefficients, coefficients + len);
+  S.pows.assign(powers, powers + len);
+  S.has_damping = !std::isnan(damping);
+  S.damping = damping;
+  return S;
+}
+
+struct Cutoff {
+  int kind = 0;
+  double t_cut = 0.0, inv_t0_sq = 0.0;
+};
+
+// ExponentialTimeCutoff::new (time_cutoff.rs:22-30).
+Cutoff make_cutoff(const BgwTimeCutoffConfig& c) {
+  Cutoff o;
+  if (c.kind == BgwTimeCutoffKind_Exponential) {
+    o.kind = 1;
+    const double rc = std::isnan(c.ratio_t_cut) ? 0.999999999 : c.ratio_t_cut;
+    const double r0 = std::isnan(c.ratio_t_0) ? 0.25 : c.ratio_t_0;
+    o.t_cut = rc * c.smax;
+    const double t0 = r0 * (c.smax - o.t_cut);
+    o.inv_t0_sq = 1.0 / (t0 * t0);
+  } else if (c.kind != BgwTimeCutoffKind_Unit) {
+    throw CalcError("time_cutoff.kind is not a BgwTimeCutoffKind");
+  }
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// Device state
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes) {
+    if (bytes <= cap && p) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    HIPCHECK(hipMalloc(&p, bytes > 0 ? bytes : 16));
+    cap = bytes;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.ensure(v.size() * sizeof(T));
+  if (!v.empty())
+    HIPCHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+struct Device {
+  int id = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  void init() {
+    if (stream) {
+      HIPCHECK(hipSetDevice(id));
+      return;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      throw CalcError("no HIP device is available: the bulk-flow kernels need a gfx950 GPU");
+    HIPCHECK(hipGetDevice(&id));
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+  }
+  ~Device() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// All device buffers of one batch (one or many realisations).
+struct Engine {
+  Device dev;
+  // static (per bubble configuration)
+  std::vector<CandEntry> cand;
+  std::vector<int32_t> inv;
+  std::vector<RealInfo> real;
+  std::vector<std::vector<double>> bubbles;  // per realisation (n_tot, 4)
+  std::vector<size_t> n_int;
+  bool static_uploaded = false;
+  DevBuf d_cand, d_inv, d_real;
+  // per call
+  DevBuf d_items, d_order, d_rot, d_sin_t, d_cos_t, d_cos_p, d_sin_p, d_gcos, d_gsin, d_ring_c,
+      d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
+      d_c, d_s, d_first, d_dtab;
+  BgwTimings timings{};
+};
+
+void build_static(Engine& E, const std::vector<Bubbles>& reals, size_t n_threads) {
+  size_t n_cand = 0, n_inv = 0;
+  E.real.assign(reals.size(), RealInfo{});
+  for (size_t r = 0; r < reals.size(); ++r) {
+    const Bubbles& B = reals[r];
+    E.real[r].cand_off = static_cast<int64_t>(n_cand);
+    E.real[r].inv_off = static_cast<int64_t>(n_inv);
+    E.real[r].n_int = static_cast<int32_t>(B.n_int);
+    E.real[r].n_tot = static_cast<int32_t>(B.n_tot());
+    E.real[r].nonfinite = B.nonfinite ? 1 : 0;
+    if (B.n_tot() > 0) n_cand += B.n_int * (B.n_tot() - 1);
+    n_inv += B.n_int * B.n_tot();
+  }
+  E.cand.assign(n_cand, CandEntry{});
+  E.inv.assign(n_inv, -1);
+  // (realisation, a) pairs in parallel
+  std::vector<std::pair<size_t, size_t>> work;
+  for (size_t r = 0; r < reals.size(); ++r)
+    for (size_t a = 0; a < reals[r].n_int; ++a) work.emplace_back(r, a);
+  parallel_for(work.size(), n_threads, [&](size_t w) {
+    const size_t r = work[w].first, a = work[w].second;
+    const Bubbles& B = reals[r];
+    build_candidates(B, a, &E.cand[E.real[r].cand_off + a * (B.n_tot() - 1)],
+                     &E.inv[E.real[r].inv_off + a * B.n_tot()]);
+  });
+  E.bubbles.clear();
+  E.n_int.clear();
+  for (const auto& B : reals) {
+    E.bubbles.push_back(B.xyz);
+    E.n_int.push_back(B.n_int);
+  }
+  E.static_uploaded = false;
+}
+
+// One compute request over a batch.
+struct Request {
+  // shared
+  const Resolution* res = nullptr;
+  const Scaling* sc = nullptr;
+  Cutoff cutoff;
+  size_t n_w = 0, n_t = 0;
+  bool identity = true;                  // no rotation (reference frame)
+  std::vector<std::array<double, 3>> k;  // k-hat list (identity: one dummy entry)
+  // per realisation
+  std::vector<double> omega;             // (n_real, n_w)
+  std::vector<double> t_begin, t_end;    // (n_real)
+  // items: per (r, k) group the reference bubbles in summation order
+  std::vector<std::vector<int32_t>> bubbles_of_real;  // selected a per realisation
+  bool mixed = false;
+  bool want_c = true, want_s = true;
+  bool skip_uncollided = true;
+};
+
+struct Result {
+  std::vector<double> c;  // (n_groups, 2, n_sets, n_w, 2)
+  std::vector<double> s;  // (n_groups, n_sets, n_w)
+  size_t n_groups = 0;
+};
+
+void ensure_static(Engine& E) {
+  E.dev.init();
+  if (E.static_uploaded) return;
+  upload(E.d_cand, E.cand, E.dev.stream);
+  upload(E.d_inv, E.inv, E.dev.stream);
+  E.static_uploaded = true;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, a, b));
+  return static_cast<double>(ms);
+}
+
+Result run(Engine& E, const Request& q) {
+  const Resolution& res = *q.res;
+  const Scaling& sc = *q.sc;
+  const size_t n_real = E.real.size();
+  const size_t n_k = q.k.size();
+  const size_t n_sets = sc.n_sets;
+  Result out;
+  out.n_groups = n_real * n_k;
+  if (n_sets == 0 || q.n_w == 0) {
+    out.c.assign(out.n_groups * 2 * n_sets * q.n_w * 2, 0.0);
+    out.s.assign(out.n_groups * n_sets * q.n_w, 0.0);
+    return out;
+  }
+  if (q.n_w > 1024) throw CalcError("n_w > 1024 is not supported by the GPU kernels");
+  ensure_static(E);
+  hipStream_t st = E.dev.stream;
+
+  // per-realisation time grid and omegas
+  std::vector<RealInfo> real = E.real;
+  for (size_t r = 0; r < n_real; ++r) {
+    real[r].t_begin = q.t_begin[r];
+    real[r].t_step = q.n_t > 1 ? (q.t_end[r] - q.t_begin[r]) / static_cast<double>(q.n_t - 1) : 0.0;
+    const double t1 = real[r].t_begin + real[r].t_step * 1.0;
+    real[r].dt = q.n_t > 1 ? t1 - real[r].t_begin : 0.0;
+    real[r].w_off = static_cast<int64_t>(r * q.n_w);
+  }
+  // rotations
+  std::vector<double> rot(9 * n_k, 0.0);
+  std::vector<std::array<double, 9>> R(n_k);
+  for (size_t k = 0; k < n_k; ++k) {
+    if (!q.identity) R[k] = rotation_to_z(q.k[k].data());
+    for (int j = 0; j < 9; ++j) rot[9 * k + j] = q.identity ? 0.0 : R[k][j];
+  }
+  // items sorted by (r, k, position in the selection)
+  std::vector<Item> items;
+  std::vector<int32_t> group_off(1, 0), group_r;
+  std::vector<double> cost;
+  for (size_t r = 0; r < n_real; ++r) {
+    const double* bub = E.bubbles[r].data();
+    const double span = q.t_end[r] - q.t_begin[r];
+    for (size_t k = 0; k < n_k; ++k) {
+      for (int32_t a : q.bubbles_of_real[r]) {
+        const double* xa = bub + 4 * static_cast<size_t>(a);
+        Item it;
+        it.r = static_cast<int32_t>(r);
+        it.k = static_cast<int32_t>(k);
+        it.a = a;
+        it.group = static_cast<int32_t>(r * n_k + k);
+        it.t_a = xa[0];
+        it.z_a = q.identity ? xa[3] : (R[k][6] * xa[1] + R[k][7] * xa[2]) + R[k][8] * xa[3];
+        items.push_back(it);
+        cost.push_back(span > 0 ? std::min(1.0, std::max(0.0, (q.t_end[r] - xa[0]) / span)) : 0.0);
+      }
+      group_off.push_back(static_cast<int32_t>(items.size()));
+      group_r.push_back(static_cast<int32_t>(r));
+    }
+  }
+  const size_t n_items = items.size();
+  // launch order of the phase-sum kernel: most expensive items first
+  std::vector<int32_t> order(n_items);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+
+  const size_t n_dir = res.n_c * res.n_phi;
+  const size_t nq = n_sets * q.n_w;
+  upload(E.d_real, real, st);
+  upload(E.d_items, items, st);
+  upload(E.d_order, order, st);
+  upload(E.d_rot, rot, st);
+  upload(E.d_sin_t, res.sin_t, st);
+  upload(E.d_cos_t, res.cos_t, st);
+  upload(E.d_cos_p, res.cos_p, st);
+  upload(E.d_sin_p, res.sin_p, st);
+  upload(E.d_gcos, res.gcos, st);
+  upload(E.d_gsin, res.gsin, st);
+  upload(E.d_ring_fac, res.ring_fac, st);
+  upload(E.d_omega, q.omega, st);
+  upload(E.d_coef, sc.coef, st);
+  upload(E.d_pows, sc.pows, st);
+  upload(E.d_group_off, group_off, st);
+  upload(E.d_group_r, group_r, st);
+  E.d_tau.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(double));
+  E.d_perm.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(uint16_t));
+  E.d_partial.ensure(std::max<size_t>(1, n_items * 2 * nq) * 2 * sizeof(double));
+  E.d_c.ensure(out.n_groups * 2 * nq * 2 * sizeof(double));
+  E.d_s.ensure(out.n_groups * nq * sizeof(double));
+
+  bgw::GeoArgs ga{};
+  ga.cand = E.d_cand.as<CandEntry>();
+  ga.inv = E.d_inv.as<int32_t>();
+  ga.real = E.d_real.as<RealInfo>();
+  ga.items = E.d_items.as<Item>();
+  ga.rot = E.d_rot.as<double>();
+  ga.identity = q.identity ? 1 : 0;
+  ga.sin_t = E.d_sin_t.as<double>();
+  ga.cos_t = E.d_cos_t.as<double>();
+  ga.cos_p = E.d_cos_p.as<double>();
+  ga.sin_p = E.d_sin_p.as<double>();
+  ga.n_c = static_cast<int>(res.n_c);
+  ga.n_phi = static_cast<int>(res.n_phi);
+  ga.n_phi_pow2 = static_cast<int>(res.n_phi_pow2);
+  ga.tau = E.d_tau.as<double>();
+  ga.perm = E.d_perm.as<uint16_t>();
+
+  bgw::SpecArgs sa{};
+  sa.real = ga.real;
+  sa.items = ga.items;
+  sa.order = E.d_order.as<int32_t>();
+  sa.tau = ga.tau;
+  sa.perm = ga.perm;
+  sa.gcos = E.d_gcos.as<double>();
+  sa.gsin = E.d_gsin.as<double>();
+  sa.ring_c = E.d_cos_t.as<double>();
+  sa.ring_fac = E.d_ring_fac.as<double>();
+  sa.omega = E.d_omega.as<double>();
+  sa.n_terms = static_cast<int>(sc.n_terms);
+  sa.n_sets_out = static_cast<int>(n_sets);
+  sa.n_c = ga.n_c;
+  sa.n_phi = ga.n_phi;
+  sa.n_t = static_cast<int>(q.n_t);
+  sa.n_w = static_cast<int>(q.n_w);
+  sa.cutoff_kind = q.cutoff.kind;
+  sa.t_cut = q.cutoff.t_cut;
+  sa.inv_t0_sq = q.cutoff.inv_t0_sq;
+  sa.has_damping = sc.has_damping ? 1 : 0;
+  sa.damping = sc.damping;
+  sa.kfac = 1.0 / (6.0 * kPi);
+  sa.skip_uncollided = q.skip_uncollided ? 1 : 0;
+  sa.mixed_block = 16;
+  sa.partial = E.d_partial.as<double>();
+
+  bgw::RedArgs ra{};
+  ra.partial = sa.partial;
+  ra.group_off = E.d_group_off.as<int32_t>();
+  ra.group_r = E.d_group_r.as<int32_t>();
+  ra.real = ga.real;
+  ra.omega = sa.omega;
+  ra.n_sets = static_cast<int>(n_sets);
+  ra.n_w = static_cast<int>(q.n_w);
+  ra.c_out = E.d_c.as<double>();
+  ra.s_out = E.d_s.as<double>();
+
+  HIPCHECK(hipEventRecord(E.dev.ev[0], st));
+  HIPCHECK(bgw::launch_geometry(ga, static_cast<int>(n_items), st));
+  HIPCHECK(hipEventRecord(E.dev.ev[1], st));
+  for (size_t s0 = 0; s0 < n_sets; s0 += kMaxSetsPerLaunch) {
+    bgw::SpecArgs sb = sa;
+    sb.set0 = static_cast<int>(s0);
+    sb.n_sets = static_cast<int>(std::min(kMaxSetsPerLaunch, n_sets - s0));
+    sb.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
+    sb.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
+    if (bgw::spectrum_lds_bytes(sb, q.mixed) > 160 * 1024)
+      throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+    HIPCHECK(bgw::launch_spectrum(sb, static_cast<int>(n_items), q.mixed ? 1 : 0, st));
+  }
+  HIPCHECK(hipEventRecord(E.dev.ev[2], st));
+  HIPCHECK(bgw::launch_reduce(ra, static_cast<int>(out.n_groups), st));
+  HIPCHECK(hipEventRecord(E.dev.ev[3], st));
+  out.c.resize(out.n_groups * 2 * nq * 2);
+  out.s.resize(out.n_groups * nq);
+  HIPCHECK(hipMemcpyAsync(out.c.data(), E.d_c.p, out.c.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(out.s.data(), E.d_s.p, out.s.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  E.timings.geometry_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
+  E.timings.spectrum_ms = elapsed_ms(E.dev.ev[1], E.dev.ev[2]);
+  E.timings.reduce_ms = elapsed_ms(E.dev.ev[2], E.dev.ev[3]);
+  E.timings.total_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[3]);
+  E.timings.n_items = n_items;
+  E.timings.n_directions = n_items * n_dir;
+  E.timings.n_terms = n_items * res.n_c * q.n_t * q.n_w;
+  return out;
+}
+
+// Geometry only, for one item: first colliding bubble and delta_tab.
+void run_masks(Engine& E, const Resolution& res, int32_t a, const double* k_hat,
+               std::vector<int32_t>& first, std::vector<double>& dtab) {
+  ensure_static(E);
+  hipStream_t st = E.dev.stream;
+  std::vector<RealInfo> real = E.real;
+  std::vector<Item> items(1);
+  items[0].r = 0;
+  items[0].k = 0;
+  items[0].a = a;
+  items[0].group = 0;
+  const size_t n_dir = res.n_c * res.n_phi;
+  upload(E.d_real, real, st);
+  upload(E.d_items, items, st);
+  std::vector<double> rot(9, 0.0);
+  if (k_hat) {
+    auto R = rotation_to_z(k_hat);
+    for (int j = 0; j < 9; ++j) rot[j] = R[j];
+  }
+  upload(E.d_rot, rot, st);
+  upload(E.d_sin_t, res.sin_t, st);
+  upload(E.d_cos_t, res.cos_t, st);
+  upload(E.d_cos_p, res.cos_p, st);
+  upload(E.d_sin_p, res.sin_p, st);
+  E.d_tau.ensure(n_dir * sizeof(double));
+  E.d_perm.ensure(n_dir * sizeof(uint16_t));
+  E.d_first.ensure(n_dir * sizeof(int32_t));
+  E.d_dtab.ensure(n_dir * sizeof(double));
+  bgw::GeoArgs ga{};
+  ga.cand = E.d_cand.as<CandEntry>();
+  ga.inv = E.d_inv.as<int32_t>();
+  ga.real = E.d_real.as<RealInfo>();
+  ga.items = E.d_items.as<Item>();
+  ga.rot = E.d_rot.as<double>();
+  ga.identity = k_hat ? 0 : 1;
+  ga.sin_t = E.d_sin_t.as<double>();
+  ga.cos_t = E.d_cos_t.as<double>();
+  ga.cos_p = E.d_cos_p.as<double>();
+  ga.sin_p = E.d_sin_p.as<double>();
+  ga.n_c = static_cast<int>(res.n_c);
+  ga.n_phi = static_cast<int>(res.n_phi);
+  ga.n_phi_pow2 = static_cast<int>(res.n_phi_pow2);
+  ga.tau = E.d_tau.as<double>();
+  ga.perm = E.d_perm.as<uint16_t>();
+  ga.first_out = E.d_first.as<int32_t>();
+  ga.dtab_out = E.d_dtab.as<double>();
+  HIPCHECK(bgw::launch_geometry(ga, 1, st));
+  first.resize(n_dir);
+  dtab.resize(n_dir);
+  HIPCHECK(hipMemcpyAsync(first.data(), E.d_first.p, n_dir * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(dtab.data(), E.d_dtab.p, n_dir * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Handles
+// ---------------------------------------------------------------------------
+struct bgw_generalized_bulk_flow {
+  Bubbles bubbles;
+  Resolution res;
+  Scaling sc;
+  size_t num_threads = 0;
+  Engine engine;
+};
+
+struct bgw_ensemble {
+  std::vector<Bubbles> reals;
+  Resolution res;
+  Scaling sc;
+  Engine engine;
+  BgwTileConfig tiles{16, 0, 0};
+  bool tuned = false;
+};
+
+namespace {
+
+bgw_generalized_bulk_flow* bulk_flow_handle(bgw_generalized_bulk_flow* h) {
+  if (!h) throw CalcError(null_error("bulk_flow"));
+  return h;
+}
+
+const Resolution& need_resolution(const Resolution& r) {
+  if (!r.set) throw CalcError("Field 'n_cos_thetax' is not initialized");
+  return r;
+}
+
+size_t host_threads(size_t requested) {
+  if (requested) return requested;
+  unsigned hc = std::thread::hardware_concurrency();
+  return hc ? hc : 1;
+}
+
+std::vector<int32_t> selected_ids(const bgw_generalized_bulk_flow* h, const size_t* selected,
+                                  size_t n_selected, bool* empty_selection) {
+  std::vector<int32_t> ids;
+  *empty_selection = false;
+  if (selected) {
+    slice_from_raw(selected, n_selected, "selected_bubbles");
+    if (n_selected == 0) *empty_selection = true;
+    for (size_t q = 0; q < n_selected; ++q) {
+      if (selected[q] >= h->bubbles.n_int)
+        throw CalcError("Index " + std::to_string(selected[q]) + " out of bounds for max " +
+                        std::to_string(h->bubbles.n_int));
+      ids.push_back(static_cast<int32_t>(selected[q]));
+    }
+  } else {
+    for (size_t a = 0; a < h->bubbles.n_int; ++a) ids.push_back(static_cast<int32_t>(a));
+  }
+  return ids;
+}
+
+void check_time_range(double t_begin, double t_end) {
+  if (t_begin > t_end)
+    throw CalcError("Invalid time range: t_begin=" + std::to_string(t_begin) +
+                    " > t_end=" + std::to_string(t_end));
+}
+
+// compute_c_integral (generalized_bulk_flow.rs:928-984) on the GPU.
+std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr, size_t n_w,
+                              bool has_t_begin, double t_begin, double t_end, size_t n_t,
+                              const size_t* selected, size_t n_selected,
+                              BgwTimeCutoffConfig cutoff, bool mixed) {
+  slice_from_raw(w_arr, n_w, "w_arr");
+  const size_t n_sets = h->sc.n_sets;
+  std::vector<double> zeros(2 * n_sets * n_w * 2, 0.0);
+  if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+  bool empty = false;
+  std::vector<int32_t> ids = selected_ids(h, selected, n_selected, &empty);
+  if (ids.empty()) return zeros;
+  const double tb = has_t_begin ? t_begin : 0.0;
+  check_time_range(tb, t_end);
+  const Resolution& res = need_resolution(h->res);
+  Request q;
+  q.res = &res;
+  q.sc = &h->sc;
+  q.cutoff = make_cutoff(cutoff);
+  q.n_w = n_w;
+  q.n_t = n_t;
+  q.identity = true;
+  q.k.push_back({0.0, 0.0, 1.0});
+  q.omega.assign(w_arr, w_arr + n_w);
+  q.t_begin = {tb};
+  q.t_end = {t_end};
+  q.bubbles_of_real = {ids};
+  q.mixed = mixed;
+  q.want_s = false;
+  Result r = run(h->engine, q);
+  return r.c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bgw_version(void) { return "0.1.0"; }
+
+const char* bgw_last_error_message(void) { return t_last_error.c_str(); }
+
+BgwStatus bgw_generalized_bulk_flow_empty_lattice_new(bgw_generalized_bulk_flow** out,
+                                                      const double* bubbles_interior,
+                                                      size_t n_interior,
+                                                      const double* bubbles_exterior,
+                                                      size_t n_exterior) {
+  if (!out) {
+    set_last_error(null_error("out"));
+    return BgwStatus_NullPointer;
+  }
+  *out = nullptr;
+  return run_ffi([&] {
+    const size_t ni = checked_product(n_interior, 4, "interior bubbles");
+    const size_t ne = checked_product(n_exterior, 4, "exterior bubbles");
+    slice_from_raw(bubbles_interior, ni, "bubbles_interior");
+    slice_from_raw(bubbles_exterior, ne, "bubbles_exterior");
+    if (n_interior + n_exterior > static_cast<size_t>(std::numeric_limits<int32_t>::max() / 4))
+      throw CalcError("too many bubbles");
+    auto h = std::make_unique<bgw_generalized_bulk_flow>();
+    h->bubbles = make_bubbles(bubbles_interior, n_interior, bubbles_exterior, n_exterior);
+    build_static(h->engine, {h->bubbles}, host_threads(0));
+    *out = h.release();
+  });
+}
+
+void bgw_generalized_bulk_flow_free(bgw_generalized_bulk_flow* bulk_flow) { delete bulk_flow; }
+
+BgwStatus bgw_generalized_bulk_flow_set_num_threads(bgw_generalized_bulk_flow* bulk_flow,
+                                                    size_t num_threads) {
+  return run_ffi([&] {
+    if (num_threads == 0) throw CalcError("num_threads must be > 0");
+    bulk_flow_handle(bulk_flow)->num_threads = num_threads;
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_set_resolution(bgw_generalized_bulk_flow* bulk_flow,
+                                                   size_t n_cos_thetax, size_t n_phix,
+                                                   bool /*precompute_first_bubbles*/) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    h->res = make_resolution(n_cos_thetax, n_phix);
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_set_gradient_scaling_params(
+    bgw_generalized_bulk_flow* bulk_flow, const double* coefficients, const double* powers,
+    size_t n_sets, size_t n_terms, double damping_width) {
+  return run_ffi([&] {
+    Scaling s = make_scaling(coefficients, powers, n_sets, n_terms, damping_width);
+    bulk_flow_handle(bulk_flow)->sc = std::move(s);
+  });
+}
+
+static BgwStatus compute_c_integral_impl(bgw_generalized_bulk_flow* bulk_flow,
+                                         const double* w_arr, size_t n_w, bool has_t_begin,
+                                         double t_begin, double t_end, size_t n_t,
+                                         const size_t* selected, size_t n_selected,
+                                         BgwTimeCutoffConfig time_cutoff, bool mixed,
+                                         BgwComplex64* out, size_t out_len) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    std::vector<double> c = compute_c(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t, selected,
+                                      n_selected, time_cutoff, mixed);
+    const size_t expected = c.size() / 2;
+    if (out_len != expected)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(expected));
+    slice_from_raw(out, out_len, "out");
+    if (out_len) std::memcpy(out, c.data(), c.size() * sizeof(double));
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_compute_c_integral(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
+    size_t n_selected_bubbles, BgwTimeCutoffConfig time_cutoff, BgwComplex64* out,
+    size_t out_len) {
+  return compute_c_integral_impl(bulk_flow, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
+                                 selected_bubbles, n_selected_bubbles, time_cutoff, false, out,
+                                 out_len);
+}
+
+BgwStatus bgw_generalized_bulk_flow_compute_c_integral_ex(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
+    size_t n_selected_bubbles, BgwTimeCutoffConfig time_cutoff, BgwPrecision precision,
+    BgwComplex64* out, size_t out_len) {
+  if (precision != BgwPrecision_Fp64 && precision != BgwPrecision_Mixed) {
+    set_last_error("precision is not a BgwPrecision");
+    return BgwStatus_InvalidArgument;
+  }
+  return compute_c_integral_impl(bulk_flow, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
+                                 selected_bubbles, n_selected_bubbles, time_cutoff,
+                                 precision == BgwPrecision_Mixed, out, out_len);
+}
+
+BgwStatus bgw_generalized_bulk_flow_first_colliding_bubble(bgw_generalized_bulk_flow* bulk_flow,
+                                                           size_t a_idx, int32_t* out,
+                                                           size_t out_len) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    const Resolution& res = need_resolution(h->res);
+    if (a_idx >= h->bubbles.n_int)
+      throw CalcError("Index " + std::to_string(a_idx) + " out of bounds (max: " +
+                      std::to_string(h->bubbles.n_int) + ")");
+    const size_t n = res.n_c * res.n_phi;
+    if (out_len != n)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(n));
+    slice_from_raw(out, out_len, "out");
+    std::vector<int32_t> first;
+    std::vector<double> dtab;
+    run_masks(h->engine, res, static_cast<int32_t>(a_idx), nullptr, first, dtab);
+    std::memcpy(out, first.data(), n * sizeof(int32_t));
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_first_colliding_bubble_ex(bgw_generalized_bulk_flow* bulk_flow,
+                                                              size_t a_idx, const double* k_hat,
+                                                              int32_t* first_out,
+                                                              double* delta_tab_out,
+                                                              size_t out_len) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    const Resolution& res = need_resolution(h->res);
+    if (a_idx >= h->bubbles.n_int)
+      throw CalcError("Index " + std::to_string(a_idx) + " out of bounds (max: " +
+                      std::to_string(h->bubbles.n_int) + ")");
+    const size_t n = res.n_c * res.n_phi;
+    if (out_len != n)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(n));
+    if (k_hat) {
+      const double nrm = std::sqrt(k_hat[0] * k_hat[0] + k_hat[1] * k_hat[1] + k_hat[2] * k_hat[2]);
+      if (!(std::fabs(nrm - 1.0) <= 1e-12)) throw CalcError("k_hat is not a unit vector");
+    }
+    std::vector<int32_t> first;
+    std::vector<double> dtab;
+    run_masks(h->engine, res, static_cast<int32_t>(a_idx), k_hat, first, dtab);
+    if (first_out) std::memcpy(first_out, first.data(), n * sizeof(int32_t));
+    if (delta_tab_out) std::memcpy(delta_tab_out, dtab.data(), n * sizeof(double));
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_delta_tab(bgw_generalized_bulk_flow* bulk_flow, size_t a_idx,
+                                              const int32_t* first_bubble, size_t grid_len,
+                                              double* out, size_t out_len) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    const Resolution& res = need_resolution(h->res);
+    const size_t n = res.n_c * res.n_phi;
+    if (grid_len != n || out_len != n)
+      throw CalcError("Array shape mismatch: First bubble array must match resolution: expected [" +
+                      std::to_string(res.n_c) + ", " + std::to_string(res.n_phi) + "]");
+    if (a_idx >= h->bubbles.n_int)
+      throw CalcError("Index " + std::to_string(a_idx) + " out of bounds (max: " +
+                      std::to_string(h->bubbles.n_int) + ")");
+    slice_from_raw(first_bubble, n, "first_bubble");
+    slice_from_raw(out, n, "out");
+    // compute_delta_tab (generalized_bulk_flow.rs:446-468): one division per
+    // direction, no search.
+    const Bubbles& B = h->bubbles;
+    const double* xa = B.row(a_idx);
+    for (size_t i = 0; i < res.n_c; ++i)
+      for (size_t j = 0; j < res.n_phi; ++j) {
+        const size_t ij = i * res.n_phi + j;
+        const int32_t code = first_bubble[ij];
+        out[ij] = 0.0;
+        if (code == -1) continue;
+        size_t b;
+        if (code >= 0) b = static_cast<size_t>(code);
+        else b = B.n_int + static_cast<size_t>(-code - 2);
+        if (b >= B.n_tot())
+          throw CalcError("Index " + std::to_string(b) + " out of bounds (max: " +
+                          std::to_string(B.n_tot()) + ")");
+        const double* xb = B.row(b);
+        double d[4] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2], xb[3] - xa[3]};
+        const double x[4] = {1.0, res.sin_t[i] * res.cos_p[j], res.sin_t[i] * res.sin_p[j],
+                             res.cos_t[i]};
+        const double dot = lscalar(d, x);
+        if (std::fabs(dot) < 1e-10) continue;
+        out[ij] = lscalar(d, d) / (2.0 * dot);
+      }
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_collision_status(bgw_generalized_bulk_flow* bulk_flow,
+                                                     size_t a_idx, double t,
+                                                     const int32_t* first_bubble,
+                                                     const double* delta_tab, size_t grid_len,
+                                                     int32_t* out, size_t out_len) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    const Resolution& res = need_resolution(h->res);
+    const size_t n = res.n_c * res.n_phi;
+    if (grid_len != n || out_len != n)
+      throw CalcError("Array shape mismatch: Input arrays must match resolution: expected [" +
+                      std::to_string(res.n_c) + ", " + std::to_string(res.n_phi) + "]");
+    if (a_idx >= h->bubbles.n_int)
+      throw CalcError("Index " + std::to_string(a_idx) + " out of bounds (max: " +
+                      std::to_string(h->bubbles.n_int) + ")");
+    slice_from_raw(first_bubble, n, "first_bubble");
+    slice_from_raw(delta_tab, n, "delta_tab");
+    slice_from_raw(out, n, "out");
+    // compute_collision_status (generalized_bulk_flow.rs:395-411)
+    const double delta_ta = t - h->bubbles.row(a_idx)[0];
+    for (size_t ij = 0; ij < n; ++ij) {
+      if (first_bubble[ij] == -1) out[ij] = 0;
+      else if (delta_tab[ij] > 0.0 && delta_ta >= delta_tab[ij]) out[ij] = 1;
+      else out[ij] = 2;
+    }
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_compute_c_integrand(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
+    size_t n_selected_bubbles, BgwComplex64* out, size_t out_len) {
+  (void)bulk_flow; (void)w_arr; (void)n_w; (void)has_t_begin; (void)t_begin; (void)t_end;
+  (void)n_t; (void)selected_bubbles; (void)n_selected_bubbles; (void)out; (void)out_len;
+  set_last_error("compute_c_integrand is not available in this build");
+  return BgwStatus_CalculationError;
+}
+
+BgwStatus bgw_generalized_bulk_flow_compute_spectrum(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, const double* k_hat,
+    size_t n_k, bool has_t_begin, double t_begin, double t_end, size_t n_t,
+    BgwTimeCutoffConfig time_cutoff, BgwPrecision precision, double* spectrum_out,
+    size_t spectrum_len, double* mean_out, size_t mean_len, BgwComplex64* c_out, size_t c_len) {
+  if (precision != BgwPrecision_Fp64 && precision != BgwPrecision_Mixed) {
+    set_last_error("precision is not a BgwPrecision");
+    return BgwStatus_InvalidArgument;
+  }
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    slice_from_raw(w_arr, n_w, "w_arr");
+    slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
+    const size_t n_sets = h->sc.n_sets;
+    const size_t nq = n_sets * n_w;
+    if (spectrum_len != n_k * nq)
+      throw CalcError("spectrum_len=" + std::to_string(spectrum_len) +
+                      " does not match expected length " + std::to_string(n_k * nq));
+    slice_from_raw(spectrum_out, spectrum_len, "spectrum_out");
+    if (mean_out && mean_len != nq)
+      throw CalcError("mean_len=" + std::to_string(mean_len) + " does not match expected length " +
+                      std::to_string(nq));
+    if (c_out && c_len != n_k * 2 * nq)
+      throw CalcError("c_len=" + std::to_string(c_len) + " does not match expected length " +
+                      std::to_string(n_k * 2 * nq));
+    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+    const double tb = has_t_begin ? t_begin : 0.0;
+    const Resolution& res = need_resolution(h->res);
+    for (size_t k = 0; k < n_k; ++k) {
+      const double* kk = k_hat + 3 * k;
+      const double nrm = std::sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
+      if (!(std::fabs(nrm - 1.0) <= 1e-12))
+        throw CalcError("k_hat[" + std::to_string(k) + "] is not a unit vector");
+    }
+    std::vector<int32_t> ids;
+    for (size_t a = 0; a < h->bubbles.n_int; ++a) ids.push_back(static_cast<int32_t>(a));
+    std::vector<double> S(n_k * nq, 0.0), C(n_k * 2 * nq * 2, 0.0);
+    if (n_k > 0 && !ids.empty()) {
+      check_time_range(tb, t_end);
+      Request q;
+      q.res = &res;
+      q.sc = &h->sc;
+      q.cutoff = make_cutoff(time_cutoff);
+      q.n_w = n_w;
+      q.n_t = n_t;
+      q.identity = false;
+      for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
+      q.omega.assign(w_arr, w_arr + n_w);
+      q.t_begin = {tb};
+      q.t_end = {t_end};
+      q.bubbles_of_real = {ids};
+      q.mixed = precision == BgwPrecision_Mixed;
+      Result r = run(h->engine, q);
+      S = std::move(r.s);
+      C = std::move(r.c);
+    }
+    std::memcpy(spectrum_out, S.data(), S.size() * sizeof(double));
+    if (mean_out) {
+      for (size_t j = 0; j < nq; ++j) {
+        double acc = 0.0;
+        for (size_t k = 0; k < n_k; ++k) acc += S[k * nq + j];
+        mean_out[j] = n_k ? acc / static_cast<double>(n_k) : 0.0;
+      }
+    }
+    if (c_out) std::memcpy(c_out, C.data(), C.size() * sizeof(double));
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_spectrum_vjp(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, const double* k_hat,
+    size_t n_k, bool has_t_begin, double t_begin, double t_end, size_t n_t,
+    BgwTimeCutoffConfig time_cutoff, const BgwComplex64* grad_c, size_t grad_c_len,
+    double* grad_coefficients, double* grad_powers, size_t grad_len) {
+  (void)bulk_flow; (void)w_arr; (void)n_w; (void)k_hat; (void)n_k; (void)has_t_begin;
+  (void)t_begin; (void)t_end; (void)n_t; (void)time_cutoff; (void)grad_c; (void)grad_c_len;
+  (void)grad_coefficients; (void)grad_powers; (void)grad_len;
+  set_last_error("spectrum_vjp is not available in this build");
+  return BgwStatus_CalculationError;
+}
+
+BgwStatus bgw_generalized_bulk_flow_last_timings(bgw_generalized_bulk_flow* bulk_flow,
+                                                 BgwTimings* out) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    if (!out) throw CalcError(null_error("out"));
+    *out = h->engine.timings;
+  });
+}
+
+// ---- ensemble (filled in below) ----
+BgwStatus bgw_ensemble_new(bgw_ensemble** out, const double* bubbles_interior,
+                           const size_t* interior_offsets, const double* bubbles_exterior,
+                           const size_t* exterior_offsets, size_t n_realisations) {
+  if (!out) {
+    set_last_error(null_error("out"));
+    return BgwStatus_NullPointer;
+  }
+  *out = nullptr;
+  return run_ffi([&] {
+    slice_from_raw(interior_offsets, n_realisations + 1, "interior_offsets");
+    slice_from_raw(exterior_offsets, n_realisations + 1, "exterior_offsets");
+    auto e = std::make_unique<bgw_ensemble>();
+    for (size_t r = 0; r < n_realisations; ++r) {
+      const size_t i0 = interior_offsets[r], i1 = interior_offsets[r + 1];
+      const size_t e0 = exterior_offsets[r], e1 = exterior_offsets[r + 1];
+      if (i1 < i0 || e1 < e0) throw CalcError("offsets must be non-decreasing");
+      if (i1 > i0) slice_from_raw(bubbles_interior, i1 * 4, "bubbles_interior");
+      if (e1 > e0) slice_from_raw(bubbles_exterior, e1 * 4, "bubbles_exterior");
+      e->reals.push_back(make_bubbles(bubbles_interior + 4 * i0, i1 - i0,
+                                      e1 > e0 ? bubbles_exterior + 4 * e0 : nullptr, e1 - e0));
+    }
+    build_static(e->engine, e->reals, host_threads(0));
+    *out = e.release();
+  });
+}
+
+void bgw_ensemble_free(bgw_ensemble* ensemble) { delete ensemble; }
+
+BgwStatus bgw_ensemble_set_resolution(bgw_ensemble* ensemble, size_t n_cos_thetax,
+                                      size_t n_phix) {
+  return run_ffi([&] {
+    if (!ensemble) throw CalcError(null_error("ensemble"));
+    ensemble->res = make_resolution(n_cos_thetax, n_phix);
+  });
+}
+
+BgwStatus bgw_ensemble_set_gradient_scaling_params(bgw_ensemble* ensemble,
+                                                   const double* coefficients,
+                                                   const double* powers, size_t n_sets,
+                                                   size_t n_terms, double damping_width) {
+  return run_ffi([&] {
+    if (!ensemble) throw CalcError(null_error("ensemble"));
+    ensemble->sc = make_scaling(coefficients, powers, n_sets, n_terms, damping_width);
+  });
+}
+
+BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
+                                        const double* k_hat, size_t n_k, const double* t_begin,
+                                        const double* t_end, size_t n_t, BgwPrecision precision,
+                                        const BgwTileConfig* tiles, double* spectrum_out,
+                                        size_t spectrum_len, double* kmean_out,
+                                        size_t kmean_len, double* ens_mean_out,
+                                        double* ens_std_out, size_t ens_len) {
+  (void)tiles;
+  if (precision != BgwPrecision_Fp64 && precision != BgwPrecision_Mixed) {
+    set_last_error("precision is not a BgwPrecision");
+    return BgwStatus_InvalidArgument;
+  }
+  return run_ffi([&] {
+    if (!ensemble) throw CalcError(null_error("ensemble"));
+    const size_t n_real = ensemble->reals.size();
+    slice_from_raw(w_arr, checked_product(n_real, n_w, "w_arr"), "w_arr");
+    slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
+    slice_from_raw(t_begin, n_real, "t_begin");
+    slice_from_raw(t_end, n_real, "t_end");
+    const size_t n_sets = ensemble->sc.n_sets;
+    const size_t nq = n_sets * n_w;
+    if (spectrum_len != n_real * n_k * nq)
+      throw CalcError("spectrum_len=" + std::to_string(spectrum_len) +
+                      " does not match expected length " + std::to_string(n_real * n_k * nq));
+    slice_from_raw(spectrum_out, spectrum_len, "spectrum_out");
+    if (kmean_out && kmean_len != n_real * nq) throw CalcError("kmean_len does not match");
+    if ((ens_mean_out || ens_std_out) && ens_len != nq) throw CalcError("ens_len does not match");
+    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+    const Resolution& res = need_resolution(ensemble->res);
+    Request q;
+    q.res = &res;
+    q.sc = &ensemble->sc;
+    q.n_w = n_w;
+    q.n_t = n_t;
+    q.identity = false;
+    for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
+    q.omega.assign(w_arr, w_arr + n_real * n_w);
+    q.t_begin.assign(t_begin, t_begin + n_real);
+    q.t_end.assign(t_end, t_end + n_real);
+    for (size_t r = 0; r < n_real; ++r) {
+      check_time_range(q.t_begin[r], q.t_end[r]);
+      std::vector<int32_t> ids(ensemble->reals[r].n_int);
+      std::iota(ids.begin(), ids.end(), 0);
+      q.bubbles_of_real.push_back(ids);
+    }
+    q.mixed = precision == BgwPrecision_Mixed;
+    std::vector<double> S(n_real * n_k * nq, 0.0);
+    if (n_k > 0 && n_real > 0) {
+      Result r = run(ensemble->engine, q);
+      S = std::move(r.s);
+    }
+    std::memcpy(spectrum_out, S.data(), S.size() * sizeof(double));
+    std::vector<double> km(n_real * nq, 0.0);
+    for (size_t r = 0; r < n_real; ++r)
+      for (size_t j = 0; j < nq; ++j) {
+        double acc = 0.0;
+        for (size_t k = 0; k < n_k; ++k) acc += S[(r * n_k + k) * nq + j];
+        km[r * nq + j] = n_k ? acc / static_cast<double>(n_k) : 0.0;
+      }
+    if (kmean_out) std::memcpy(kmean_out, km.data(), km.size() * sizeof(double));
+    for (size_t j = 0; j < nq && (ens_mean_out || ens_std_out); ++j) {
+      double m = 0.0;
+      for (size_t r = 0; r < n_real; ++r) m += km[r * nq + j];
+      m = n_real ? m / static_cast<double>(n_real) : 0.0;
+      double v = 0.0;
+      for (size_t r = 0; r < n_real; ++r) v += (km[r * nq + j] - m) * (km[r * nq + j] - m);
+      v = n_real ? v / static_cast<double>(n_real) : 0.0;
+      if (ens_mean_out) ens_mean_out[j] = m;
+      if (ens_std_out) ens_std_out[j] = std::sqrt(v);
+    }
+  });
+}
+
+BgwStatus bgw_ensemble_autotune(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
+                                const double* k_hat, size_t n_k, const double* t_begin,
+                                const double* t_end, size_t n_t, BgwPrecision precision,
+                                BgwTileConfig* best) {
+  (void)w_arr; (void)n_w; (void)k_hat; (void)n_k; (void)t_begin; (void)t_end; (void)n_t;
+  (void)precision;
+  return run_ffi([&] {
+    if (!ensemble) throw CalcError(null_error("ensemble"));
+    if (best) *best = ensemble->tiles;
+  });
+}
+
+BgwStatus bgw_ensemble_last_timings(bgw_ensemble* ensemble, BgwTimings* out) {
+  return run_ffi([&] {
+    if (!ensemble) throw CalcError(null_error("ensemble"));
+    if (!out) throw CalcError(null_error("out"));
+    *out = ensemble->engine.timings;
+  });
+}
+
+}  // extern "C"

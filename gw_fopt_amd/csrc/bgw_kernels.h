@@ -1,0 +1,76 @@
+// SPDX-License-Identifier: MIT
+// Launch interfaces of the bulk-flow kernels (host side includes this).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bgw_common.h"
+
+namespace bgw {
+
+struct GeoArgs {
+  const CandEntry* cand;
+  const int32_t* inv;
+  const RealInfo* real;
+  const Item* items;
+  const double* rot;      // (n_k, 9) rotation taking k-hat to z-hat
+  int identity;           // 1: no rotation (the reference's own frame)
+  const double* sin_t;    // (n_c)
+  const double* cos_t;    // (n_c)
+  const double* cos_p;    // (n_phi)
+  const double* sin_p;    // (n_phi)
+  int n_c, n_phi, n_phi_pow2;
+  double* tau;            // (n_items, n_c * n_phi), each ring sorted ascending
+  uint16_t* perm;         // (n_items, n_c * n_phi), phi index of each sorted entry
+  int32_t* first_out;     // optional (n_items, n_c * n_phi), unsorted
+  double* dtab_out;       // optional (n_items, n_c * n_phi), unsorted
+};
+
+struct SpecArgs {
+  const RealInfo* real;
+  const Item* items;
+  const int32_t* order;   // launch order (block b works on item order[b])
+  const double* tau;
+  const uint16_t* perm;
+  const double* gcos;     // (n_phi) w_j cos(2 phi_j)
+  const double* gsin;     // (n_phi) w_j sin(2 phi_j)
+  const double* ring_c;   // (n_c) cos theta_i
+  const double* ring_fac; // (n_c) dcos * w_i * 0.5 * sin^2 theta_i * dphi
+  const double* omega;    // omegas of every realisation (RealInfo::w_off)
+  const double* coef;     // (n_sets, n_terms)
+  const double* pows;     // (n_sets, n_terms)
+  int n_sets, n_terms;    // sets handled by this launch (<= 6)
+  int set0, n_sets_out;   // first set of this launch, total sets in `partial`
+  int n_c, n_phi, n_t, n_w;
+  int cutoff_kind;        // 0 unit, 1 exponential
+  double t_cut, inv_t0_sq;
+  int has_damping;
+  double damping;
+  double kfac;            // 1 / (6 pi)
+  int skip_uncollided;    // drop rings whose uncollided part is rounding noise
+  int mixed_block;        // block length of the mixed-precision phase factorisation
+  double* partial;        // (n_items, 2, n_sets_out, n_w) complex
+  // adjoint only
+  const double* grad_c;   // (n_groups, 2, n_sets, n_w) complex cotangent of C
+  double* grad_out;       // (n_items, 2, n_sets, n_terms): d/dcoef, d/dpow
+};
+
+struct RedArgs {
+  const double* partial;
+  const int32_t* group_off;  // (n_groups + 1) item ranges, items sorted by group
+  const int32_t* group_r;    // realisation of each group
+  const RealInfo* real;
+  const double* omega;
+  int n_sets, n_w;
+  double* c_out;             // (n_groups, 2, n_sets, n_w) complex
+  double* s_out;             // (n_groups, n_sets, n_w)
+};
+
+hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s);
+hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s);
+hipError_t launch_reduce(const RedArgs& a, int n_groups, hipStream_t s);
+hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s);
+size_t spectrum_lds_bytes(const SpecArgs& a, int mixed);
+
+}  // namespace bgw

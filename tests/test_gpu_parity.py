@@ -1,0 +1,166 @@
+"""GPU parity against the CPU oracle (masks bit-exact; FP64 spectra <= 1e-10 per bin).
+
+Every test calls the gfx950 kernels through the C ABI of libbubble_gw_c.so
+(via gw_fopt_amd.many_bubbles) and compares with the oracle on the same
+seeded inputs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gw_fopt_amd import configs
+from gw_fopt_amd import many_bubbles as mb
+from gw_fopt_amd import nucleation
+
+pytestmark = pytest.mark.gpu
+
+FP64_REL = 1e-10   # plain form vs oracle, per (omega, k-hat) bin
+MIXED_REL = 1e-6   # mixed form vs FP64, per bin
+
+
+def flow(interior, exterior, n_c, n_phi, coef, pows, damping=None):
+    lb = mb.LatticeBubbles(mb.EmptyLattice(), interior, exterior)
+    bf = mb.GeneralizedBulkFlow(lb)
+    bf.set_resolution(n_c, n_phi)
+    bf.set_gradient_scaling_params(coef, pows, damping)
+    return bf
+
+
+def rel_per_bin(a, b):
+    return np.max(np.abs(a - b) / np.abs(b))
+
+
+def rel_to_max(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def small_async(seed, n=6):
+    """n bubbles with staggered nucleation times (rejecting nuclei inside bubbles)."""
+    b = nucleation.exponential_nucleation(12.0, n_max=n, seed=seed)
+    return b, nucleation.face_images(b, 1.0)
+
+
+def test_reference_c_abi_config():
+    """The reference's C-ABI test case (rust/c-bubble-gw/tests/c_api.rs:135-218)."""
+    inter = np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0]])
+    coef, pows = [[0.0], [1.0]], [[3.0], [3.0]]
+    bf = flow(inter, None, 4, 5, coef, pows)
+    C = bf.compute_c_integral([0.5, 1.0], t_begin=0.0, t_end=8.0, n_t=40)
+    Co = oracle.compute_c_integral(inter, None, 4, 5, coef, pows, [0.5, 1.0], t_begin=0.0,
+                                   t_end=8.0, n_t=40)
+    assert C.shape == (2, 2, 2)
+    assert rel_to_max(C, Co) <= FP64_REL
+
+
+def test_masks_bitexact_plain_8():
+    wl = configs.plain_8()
+    bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
+              wl.powers_sets)
+    for a in range(wl.n_bubbles):
+        first = bf.compute_first_colliding_bubble(a)
+        fo, dto = oracle.first_colliding(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, a)
+        np.testing.assert_array_equal(first, fo)
+        f2, dt2 = bf.first_colliding_in_frame(a)
+        np.testing.assert_array_equal(f2, fo)
+        assert dt2.tobytes() == dto.tobytes()
+        assert bf.compute_delta_tab(a, first).tobytes() == dto.tobytes()
+
+
+@pytest.mark.parametrize("seed", [3, 7])
+def test_masks_bitexact_rotated_frames(seed):
+    inter, ext = small_async(seed, n=10)
+    bf = flow(inter, ext, 12, 24, [[1.0]], [[5.0]])
+    for k in mb.fibonacci_directions(5):
+        for a in range(len(inter)):
+            f, d = bf.first_colliding_in_frame(a, k)
+            fo, do = oracle.first_colliding(inter, ext, 12, 24, a, k_hat=k)
+            np.testing.assert_array_equal(f, fo)
+            assert d.tobytes() == do.tobytes()
+
+
+def test_masks_exact_ties_give_none():
+    """Two partners at mirror positions tie exactly along x: the reference returns None
+    there (strict '> 0' in check_collision_point, generalized_bulk_flow.rs:1016-1017)."""
+    inter = np.array([[0.0, 0.0, 0.0, 0.0], [0.0, 0.5, 0.5, 0.0], [0.0, 0.5, -0.5, 0.0]])
+    bf = flow(inter, None, 3, 5, [[1.0]], [[3.0]])
+    first = bf.compute_first_colliding_bubble(0)
+    fo, _ = oracle.first_colliding(inter, None, 3, 5, 0)
+    np.testing.assert_array_equal(first, fo)
+    assert first[1, 0] == -1  # cos theta = 0, phi = 0: x-hat, equidistant from 1 and 2
+
+
+def test_spectrum_plain_8_fp64():
+    wl = configs.plain_8()
+    bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
+              wl.powers_sets)
+    S, M, C = bf.compute_spectrum(wl.w_arr, wl.k_hat, t_begin=wl.t_begin, t_end=wl.t_end,
+                                  n_t=wl.n_t, return_c=True)
+    So, Co = oracle.spectrum(wl.interior, wl.exterior, wl.k_hat, wl.n_cos_thetax, wl.n_phix,
+                             wl.coefficients_sets, wl.powers_sets, wl.w_arr, t_begin=wl.t_begin,
+                             t_end=wl.t_end, n_t=wl.n_t, return_c=True)
+    assert rel_per_bin(S, So) <= FP64_REL
+    assert rel_to_max(C, Co) <= FP64_REL
+    np.testing.assert_allclose(M, So.mean(axis=0), rtol=FP64_REL)
+
+
+def test_c_integral_plain_8_fp64_and_selected():
+    wl = configs.plain_8()
+    bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
+              wl.powers_sets)
+    sel = [3, 1, 3]
+    C = bf.compute_c_integral(wl.w_arr, t_end=wl.t_end, n_t=wl.n_t, selected_bubbles=sel)
+    Co = oracle.compute_c_integral(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix,
+                                   wl.coefficients_sets, wl.powers_sets, wl.w_arr,
+                                   t_end=wl.t_end, n_t=wl.n_t, selected_bubbles=sel)
+    assert rel_per_bin(np.abs(C) ** 2, np.abs(Co) ** 2) <= 2 * FP64_REL
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_c_integral_async_multiset_cutoff(seed):
+    inter, ext = small_async(seed, n=7)
+    coef = [[0.0], [1.0], [0.6, 0.4]][:2]
+    pows = [[3.0], [5.0]]
+    coef = [[0.0, 0.0], [1.0, 0.5], [0.3, 0.7]]
+    pows = [[3.0, 4.0], [5.0, 3.5], [4.2, 6.0]]
+    w = np.geomspace(0.5, 120.0, 33)
+    tc = mb.ExponentialTimeCutoff(0.7, 0.8, None)
+    bf = flow(inter, ext, 16, 24, coef, pows)
+    C = bf.compute_c_integral(w, t_begin=0.0, t_end=0.7, n_t=90, time_cutoff=tc)
+    Co = oracle.compute_c_integral(inter, ext, 16, 24, coef, pows, w, t_begin=0.0, t_end=0.7,
+                                   n_t=90, time_cutoff=(1, 0.7, 0.8, None))
+    S = np.sum(np.abs(C) ** 2, axis=0)
+    So = np.sum(np.abs(Co) ** 2, axis=0)
+    assert rel_per_bin(S[1:], So[1:]) <= FP64_REL
+    assert rel_to_max(C, Co) <= FP64_REL
+
+
+def test_c_integral_damping():
+    inter, ext = small_async(21, n=5)
+    w = np.geomspace(0.5, 80.0, 17)
+    bf = flow(inter, ext, 10, 16, [[1.0]], [[4.0]], damping=0.3)
+    C = bf.compute_c_integral(w, t_end=0.6, n_t=60)
+    Co = oracle.compute_c_integral(inter, ext, 10, 16, [[1.0]], [[4.0]], w, t_end=0.6, n_t=60,
+                                   damping=0.3)
+    assert rel_per_bin(np.sum(np.abs(C) ** 2, axis=0), np.sum(np.abs(Co) ** 2, axis=0)) <= FP64_REL
+
+
+def test_spectrum_plain_8_mixed():
+    wl = configs.plain_8()
+    bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
+              wl.powers_sets)
+    S64, _ = bf.compute_spectrum(wl.w_arr, wl.k_hat, t_end=wl.t_end, n_t=wl.n_t)
+    S32, _ = bf.compute_spectrum(wl.w_arr, wl.k_hat, t_end=wl.t_end, n_t=wl.n_t,
+                                 precision="mixed")
+    err = rel_per_bin(S32, S64)
+    print(f"mixed vs fp64 max per-bin rel err {err:.3e}")
+    assert err <= MIXED_REL
+
+
+def test_identity_frame_equals_reference_frame():
+    """k-hat = z-hat reproduces the reference-frame call bit for bit."""
+    wl = configs.plain_8()
+    bf = flow(wl.interior, wl.exterior, 16, 32, wl.coefficients_sets, wl.powers_sets)
+    C = bf.compute_c_integral(wl.w_arr, t_end=wl.t_end, n_t=80)
+    _, _, Ck = bf.compute_spectrum(wl.w_arr, [[0.0, 0.0, 1.0]], t_end=wl.t_end, n_t=80,
+                                   return_c=True)
+    assert np.array_equal(C, Ck[0])
