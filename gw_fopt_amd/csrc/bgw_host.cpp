@@ -122,6 +122,20 @@ std::array<double, 9> rotation_to_z(const double* k) {
   return {ct * cp, ct * sp, -st, -sp, cp, 0.0, k[0], k[1], k[2]};
 }
 
+bool valid_precision(BgwPrecision p) {
+  return p == BgwPrecision_Fp64 || p == BgwPrecision_Mixed || p == BgwPrecision_Fp64Blocked;
+}
+
+// Phase-sum kernel variant of a precision.
+int kernel_mode(BgwPrecision p) {
+  switch (p) {
+    case BgwPrecision_Fp64: return 0;
+    case BgwPrecision_Fp64Blocked: return 1;
+    case BgwPrecision_Mixed: return 2;
+  }
+  throw CalcError("precision is not a BgwPrecision");
+}
+
 std::string bubble_index_name(size_t b, size_t n_int) {
   return b < n_int ? "Interior(" + std::to_string(b) + ")"
                    : "Exterior(" + std::to_string(b - n_int) + ")";
@@ -440,7 +454,7 @@ struct Request {
   std::vector<double> t_begin, t_end;    // (n_real)
   // items: per (r, k) group the reference bubbles in summation order
   std::vector<std::vector<int32_t>> bubbles_of_real;  // selected a per realisation
-  bool mixed = false;
+  int mode = 0;  // phase-sum kernel: 0 FP64 direct, 1 FP64 blocked, 2 mixed
   bool want_c = true, want_s = true;
   bool skip_uncollided = true;
 };
@@ -617,9 +631,9 @@ Result run(Engine& E, const Request& q) {
     sb.n_sets = static_cast<int>(std::min(kMaxSetsPerLaunch, n_sets - s0));
     sb.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
     sb.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
-    if (bgw::spectrum_lds_bytes(sb, q.mixed) > 160 * 1024)
+    if (bgw::spectrum_lds_bytes(sb, q.mode) > 160 * 1024)
       throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
-    HIPCHECK(bgw::launch_spectrum(sb, static_cast<int>(n_items), q.mixed ? 1 : 0, st));
+    HIPCHECK(bgw::launch_spectrum(sb, static_cast<int>(n_items), q.mode, st));
   }
   HIPCHECK(hipEventRecord(E.dev.ev[2], st));
   HIPCHECK(bgw::launch_reduce(ra, static_cast<int>(out.n_groups), st));
@@ -766,7 +780,7 @@ void check_time_range(double t_begin, double t_end) {
 std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr, size_t n_w,
                               bool has_t_begin, double t_begin, double t_end, size_t n_t,
                               const size_t* selected, size_t n_selected,
-                              BgwTimeCutoffConfig cutoff, bool mixed) {
+                              BgwTimeCutoffConfig cutoff, int mode) {
   slice_from_raw(w_arr, n_w, "w_arr");
   const size_t n_sets = h->sc.n_sets;
   std::vector<double> zeros(2 * n_sets * n_w * 2, 0.0);
@@ -789,7 +803,7 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
   q.t_begin = {tb};
   q.t_end = {t_end};
   q.bubbles_of_real = {ids};
-  q.mixed = mixed;
+  q.mode = mode;
   q.want_s = false;
   Result r = run(h->engine, q);
   return r.c;
@@ -859,12 +873,12 @@ static BgwStatus compute_c_integral_impl(bgw_generalized_bulk_flow* bulk_flow,
                                          const double* w_arr, size_t n_w, bool has_t_begin,
                                          double t_begin, double t_end, size_t n_t,
                                          const size_t* selected, size_t n_selected,
-                                         BgwTimeCutoffConfig time_cutoff, bool mixed,
+                                         BgwTimeCutoffConfig time_cutoff, int mode,
                                          BgwComplex64* out, size_t out_len) {
   return run_ffi([&] {
     auto* h = bulk_flow_handle(bulk_flow);
     std::vector<double> c = compute_c(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t, selected,
-                                      n_selected, time_cutoff, mixed);
+                                      n_selected, time_cutoff, mode);
     const size_t expected = c.size() / 2;
     if (out_len != expected)
       throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
@@ -880,7 +894,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integral(
     size_t n_selected_bubbles, BgwTimeCutoffConfig time_cutoff, BgwComplex64* out,
     size_t out_len) {
   return compute_c_integral_impl(bulk_flow, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
-                                 selected_bubbles, n_selected_bubbles, time_cutoff, false, out,
+                                 selected_bubbles, n_selected_bubbles, time_cutoff, 0, out,
                                  out_len);
 }
 
@@ -889,13 +903,13 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integral_ex(
     double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
     size_t n_selected_bubbles, BgwTimeCutoffConfig time_cutoff, BgwPrecision precision,
     BgwComplex64* out, size_t out_len) {
-  if (precision != BgwPrecision_Fp64 && precision != BgwPrecision_Mixed) {
+  if (!valid_precision(precision)) {
     set_last_error("precision is not a BgwPrecision");
     return BgwStatus_InvalidArgument;
   }
   return compute_c_integral_impl(bulk_flow, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
                                  selected_bubbles, n_selected_bubbles, time_cutoff,
-                                 precision == BgwPrecision_Mixed, out, out_len);
+                                 kernel_mode(precision), out, out_len);
 }
 
 BgwStatus bgw_generalized_bulk_flow_first_colliding_bubble(bgw_generalized_bulk_flow* bulk_flow,
@@ -1031,7 +1045,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_spectrum(
     size_t n_k, bool has_t_begin, double t_begin, double t_end, size_t n_t,
     BgwTimeCutoffConfig time_cutoff, BgwPrecision precision, double* spectrum_out,
     size_t spectrum_len, double* mean_out, size_t mean_len, BgwComplex64* c_out, size_t c_len) {
-  if (precision != BgwPrecision_Fp64 && precision != BgwPrecision_Mixed) {
+  if (!valid_precision(precision)) {
     set_last_error("precision is not a BgwPrecision");
     return BgwStatus_InvalidArgument;
   }
@@ -1077,7 +1091,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_spectrum(
       q.t_begin = {tb};
       q.t_end = {t_end};
       q.bubbles_of_real = {ids};
-      q.mixed = precision == BgwPrecision_Mixed;
+      q.mode = kernel_mode(precision);
       Result r = run(h->engine, q);
       S = std::move(r.s);
       C = std::move(r.c);
@@ -1170,7 +1184,7 @@ BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_
                                         size_t kmean_len, double* ens_mean_out,
                                         double* ens_std_out, size_t ens_len) {
   (void)tiles;
-  if (precision != BgwPrecision_Fp64 && precision != BgwPrecision_Mixed) {
+  if (!valid_precision(precision)) {
     set_last_error("precision is not a BgwPrecision");
     return BgwStatus_InvalidArgument;
   }
@@ -1207,7 +1221,7 @@ BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_
       std::iota(ids.begin(), ids.end(), 0);
       q.bubbles_of_real.push_back(ids);
     }
-    q.mixed = precision == BgwPrecision_Mixed;
+    q.mode = kernel_mode(precision);
     std::vector<double> S(n_real * n_k * nq, 0.0);
     if (n_k > 0 && n_real > 0) {
       Result r = run(ensemble->engine, q);

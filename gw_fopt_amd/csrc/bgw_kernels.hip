@@ -31,6 +31,8 @@
 //   (deterministic) and form S = 8 pi w^3 (|C+|^2 + |Cx|^2).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bgw_common.h"
 #include "bgw_device.h"
 #include "bgw_kernels.h"
@@ -255,15 +257,20 @@ __global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
       }
     }
     __syncthreads();
-    // Bitonic sort of every P-slot segment (one ring each), ascending.
+    // Bitonic sort of every P-slot segment (one ring each), ascending in
+    // (delay, phi index).  Padding slots carry phi index >= n_phi, so they
+    // sort after the directions that never collide (delay +inf as well).
     for (int k = 2; k <= P; k <<= 1) {
       for (int jj = k >> 1; jj > 0; jj >>= 1) {
         for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
           const int p = e ^ jj;
           if (p > e) {
-            const bool up = (e & k) == 0;
+            // direction from the index within the ring's segment, so every
+            // segment ends ascending (k == P merges the whole segment)
+            const bool up = ((e & (P - 1)) & k) == 0;
             const double ke = sort_key[e], kp = sort_key[p];
-            if ((ke > kp) == up && ke != kp) {
+            const bool greater = ke > kp || (ke == kp && sort_val[e] > sort_val[p]);
+            if (greater == up) {
               sort_key[e] = kp;
               sort_key[p] = ke;
               const int32_t t = sort_val[e];
@@ -493,8 +500,16 @@ __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0
 // ---------------------------------------------------------------------------
 // Stage 2: phase sums
 // ---------------------------------------------------------------------------
-template <int NS, int OPL, bool MIXED, int RB>
+// MODE 0: FP64, one double sincos per (t, omega) term (the parity form).
+// MODE 1: FP64, block-factorised phases: exp(i w psi(t0 + r)) = A(t0) E_r with
+//         E_r = exp(i w r dt (1 - cos th)); double E_r (recurrence), double
+//         anchors A, double block sums.
+// MODE 2: mixed, the same factorisation with the phase arguments reduced in
+//         double and the sincos, E_r table, weights and block sums in float;
+//         block results accumulated in float with Kahan compensation.
+template <int NS, int OPL, int MODE, int RB>
 __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
+  using Acc = typename std::conditional<MODE == 2, float, double>::type;
   extern __shared__ double smem[];
   const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
   const Item it = g.items[item_id];
@@ -507,19 +522,21 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
     const int wi = threadIdx.x + q * blockDim.x;
     om[q] = wi < g.n_w ? g.omega[ri.w_off + wi] : 0.0;
   }
-  // FP64 accumulators, or float sums + Kahan compensations.
-  double acc[OPL][NS][4];
-  float fs[OPL][NS][4], fc[OPL][NS][4];
+  // running sums (double), or float sums + Kahan compensations in MODE 2
+  Acc acc[OPL][NS][4];
+  float comp[MODE == 2 ? OPL : 1][MODE == 2 ? NS : 1][4];
 #pragma unroll
   for (int q = 0; q < OPL; ++q)
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        acc[q][s][c] = 0.0;
-        fs[q][s][c] = 0.0f;
-        fc[q][s][c] = 0.0f;
-      }
+      for (int c = 0; c < 4; ++c) acc[q][s][c] = Acc(0);
+#pragma unroll
+  for (int q = 0; q < (MODE == 2 ? OPL : 1); ++q)
+#pragma unroll
+    for (int s = 0; s < (MODE == 2 ? NS : 1); ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) comp[q][s][c] = 0.0f;
 
   for (int i = 0; i < g.n_c; ++i) {
     if (g.ring_fac[i] == 0.0) continue;  // sin^2 theta = 0: B = 0 exactly
@@ -529,9 +546,10 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
       __syncthreads();
       continue;
     }
-    // mixed form: E_r = exp(i omega r step (1 - cos th)), r < RB
-    float er[MIXED ? OPL : 1][MIXED ? RB : 1], ei[MIXED ? OPL : 1][MIXED ? RB : 1];
-    if constexpr (MIXED) {
+    // E_r = exp(i omega r step (1 - cos th)), r < RB, by a double recurrence
+    constexpr int kE = MODE == 0 ? 1 : RB;
+    Acc er[OPL][kE], ei[OPL][kE];
+    if constexpr (MODE != 0) {
       const double delta = ri.t_step * (1.0 - g.ring_c[i]);
 #pragma unroll
       for (int q = 0; q < OPL; ++q) {
@@ -540,8 +558,8 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
         double cr = 1.0, ci = 0.0;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
-          er[q][r] = static_cast<float>(cr);
-          ei[q][r] = static_cast<float>(ci);
+          er[q][r] = static_cast<Acc>(cr);
+          ei[q][r] = static_cast<Acc>(ci);
           const double nr = cr * c1 - ci * s1;
           const double ni = cr * s1 + ci * c1;
           cr = nr;
@@ -553,7 +571,7 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
       const int len = min(kTimeChunk, g.n_t - t0);
       prepare_times(g, L, i, t0, len, it, ri);
       __syncthreads();
-      if constexpr (!MIXED) {
+      if constexpr (MODE == 0) {
         for (int l = 0; l < len; ++l) {
           const double psi = L.psi[l];
           const double* v = L.v + l * 2 * NS;
@@ -574,40 +592,55 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
       } else {
         for (int l0 = 0; l0 < len; l0 += RB) {
           const double psi0 = L.psi[l0];
-          float bs[OPL][NS][4];
+          Acc bs[OPL][NS][4];
 #pragma unroll
           for (int q = 0; q < OPL; ++q)
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
-              for (int c = 0; c < 4; ++c) bs[q][s][c] = 0.0f;
+              for (int c = 0; c < 4; ++c) bs[q][s][c] = Acc(0);
 #pragma unroll
           for (int r = 0; r < RB; ++r) {
             const double* v = L.v + (l0 + r) * 2 * NS;  // zero-padded past len
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-              const float vp = static_cast<float>(v[2 * s]);
-              const float vx = static_cast<float>(v[2 * s + 1]);
+              const Acc vp = static_cast<Acc>(v[2 * s]);
+              const Acc vx = static_cast<Acc>(v[2 * s + 1]);
 #pragma unroll
               for (int q = 0; q < OPL; ++q) {
-                bs[q][s][0] = __builtin_fmaf(vp, er[q][r], bs[q][s][0]);
-                bs[q][s][1] = __builtin_fmaf(vp, ei[q][r], bs[q][s][1]);
-                bs[q][s][2] = __builtin_fmaf(vx, er[q][r], bs[q][s][2]);
-                bs[q][s][3] = __builtin_fmaf(vx, ei[q][r], bs[q][s][3]);
+                bs[q][s][0] = fma(vp, er[q][r], bs[q][s][0]);
+                bs[q][s][1] = fma(vp, ei[q][r], bs[q][s][1]);
+                bs[q][s][2] = fma(vx, er[q][r], bs[q][s][2]);
+                bs[q][s][3] = fma(vx, ei[q][r], bs[q][s][3]);
               }
             }
           }
 #pragma unroll
           for (int q = 0; q < OPL; ++q) {
-            float as, ac;
-            sincos_f64_to_f32(om[q] * psi0, &as, &ac);
+            Acc as, ac;
+            if constexpr (MODE == 2) {
+              sincos_f64_to_f32(om[q] * psi0, &as, &ac);
+            } else {
+              sincos_f64(om[q] * psi0, &as, &ac);
+            }
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-              // (ac + i as) * (b0 + i b1), likewise for x
-              kahan_add(fs[q][s][0], fc[q][s][0], __builtin_fmaf(ac, bs[q][s][0], -as * bs[q][s][1]));
-              kahan_add(fs[q][s][1], fc[q][s][1], __builtin_fmaf(ac, bs[q][s][1], as * bs[q][s][0]));
-              kahan_add(fs[q][s][2], fc[q][s][2], __builtin_fmaf(ac, bs[q][s][2], -as * bs[q][s][3]));
-              kahan_add(fs[q][s][3], fc[q][s][3], __builtin_fmaf(ac, bs[q][s][3], as * bs[q][s][2]));
+              // (ac + i as) * (b0 + i b1), likewise for the cross polarisation
+              const Acc p0 = fma(ac, bs[q][s][0], -as * bs[q][s][1]);
+              const Acc p1 = fma(ac, bs[q][s][1], as * bs[q][s][0]);
+              const Acc p2 = fma(ac, bs[q][s][2], -as * bs[q][s][3]);
+              const Acc p3 = fma(ac, bs[q][s][3], as * bs[q][s][2]);
+              if constexpr (MODE == 2) {
+                kahan_add(acc[q][s][0], comp[q][s][0], p0);
+                kahan_add(acc[q][s][1], comp[q][s][1], p1);
+                kahan_add(acc[q][s][2], comp[q][s][2], p2);
+                kahan_add(acc[q][s][3], comp[q][s][3], p3);
+              } else {
+                acc[q][s][0] += p0;
+                acc[q][s][1] += p1;
+                acc[q][s][2] += p2;
+                acc[q][s][3] += p3;
+              }
             }
           }
         }
@@ -624,24 +657,21 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
     if (wi >= g.n_w) continue;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      double c0, c1, c2, c3;
-      if constexpr (MIXED) {
-        c0 = static_cast<double>(fs[q][s][0]) - static_cast<double>(fc[q][s][0]);
-        c1 = static_cast<double>(fs[q][s][1]) - static_cast<double>(fc[q][s][1]);
-        c2 = static_cast<double>(fs[q][s][2]) - static_cast<double>(fc[q][s][2]);
-        c3 = static_cast<double>(fs[q][s][3]) - static_cast<double>(fc[q][s][3]);
-      } else {
-        c0 = acc[q][s][0];
-        c1 = acc[q][s][1];
-        c2 = acc[q][s][2];
-        c3 = acc[q][s][3];
+      double c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (MODE == 2) {
+          c[j] = static_cast<double>(acc[q][s][j]) - static_cast<double>(comp[q][s][j]);
+        } else {
+          c[j] = static_cast<double>(acc[q][s][j]);
+        }
       }
       const int64_t ip = (static_cast<int64_t>(g.set0 + s) * g.n_w + wi) * 2;
       const int64_t ix = ((nq + static_cast<int64_t>(g.set0 + s) * g.n_w) + wi) * 2;
-      out[ip] = c0;
-      out[ip + 1] = c1;
-      out[ix] = c2;
-      out[ix + 1] = c3;
+      out[ip] = c[0];
+      out[ip + 1] = c[1];
+      out[ix] = c[2];
+      out[ix + 1] = c[3];
     }
   }
 }
@@ -697,10 +727,10 @@ size_t spectrum_lds_bytes(const SpecArgs& a, int /*mixed*/) {
 }
 
 namespace {
-template <int NS, int OPL, bool MIXED>
+template <int NS, int OPL, int MODE>
 hipError_t launch_spec_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
   constexpr int RB = 16;
-  auto k = spectrum_kernel<NS, OPL, MIXED, RB>;
+  auto k = spectrum_kernel<NS, OPL, MODE, RB>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
@@ -709,40 +739,44 @@ hipError_t launch_spec_t(const SpecArgs& a, int n_items, int threads, size_t lds
   return hipGetLastError();
 }
 
-template <int NS>
-hipError_t launch_spec_ns(const SpecArgs& a, int n_items, int mixed, int opl, int threads,
-                          size_t lds, hipStream_t s) {
-  if (mixed) {
-    switch (opl) {
-      case 1: return launch_spec_t<NS, 1, true>(a, n_items, threads, lds, s);
-      case 2: return launch_spec_t<NS, 2, true>(a, n_items, threads, lds, s);
-      default: return launch_spec_t<NS, 4, true>(a, n_items, threads, lds, s);
-    }
-  }
+template <int NS, int MODE>
+hipError_t launch_spec_opl(const SpecArgs& a, int n_items, int opl, int threads, size_t lds,
+                           hipStream_t s) {
   switch (opl) {
-    case 1: return launch_spec_t<NS, 1, false>(a, n_items, threads, lds, s);
-    case 2: return launch_spec_t<NS, 2, false>(a, n_items, threads, lds, s);
-    default: return launch_spec_t<NS, 4, false>(a, n_items, threads, lds, s);
+    case 1: return launch_spec_t<NS, 1, MODE>(a, n_items, threads, lds, s);
+    case 2: return launch_spec_t<NS, 2, MODE>(a, n_items, threads, lds, s);
+    default: return launch_spec_t<NS, 4, MODE>(a, n_items, threads, lds, s);
+  }
+}
+
+template <int NS>
+hipError_t launch_spec_ns(const SpecArgs& a, int n_items, int mode, int opl, int threads,
+                          size_t lds, hipStream_t s) {
+  switch (mode) {
+    case 0: return launch_spec_opl<NS, 0>(a, n_items, opl, threads, lds, s);
+    case 1: return launch_spec_opl<NS, 1>(a, n_items, opl, threads, lds, s);
+    default: return launch_spec_opl<NS, 2>(a, n_items, opl, threads, lds, s);
   }
 }
 }  // namespace
 
-hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s) {
+hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t s) {
   if (n_items == 0) return hipSuccess;
+  if (mode < 0 || mode > 2) return hipErrorInvalidValue;
   // lanes own omegas: 64..256 threads, up to 4 omegas per lane
   int threads = ((a.n_w + 63) / 64) * 64;
   if (threads > 256) threads = 256;
   const int opl_needed = (a.n_w + threads - 1) / threads;
   const int opl = opl_needed <= 1 ? 1 : (opl_needed <= 2 ? 2 : 4);
   if (opl_needed > 4) return hipErrorInvalidValue;
-  const size_t lds = spectrum_lds_bytes(a, mixed);
+  const size_t lds = spectrum_lds_bytes(a, mode);
   switch (a.n_sets) {
-    case 1: return launch_spec_ns<1>(a, n_items, mixed, opl, threads, lds, s);
-    case 2: return launch_spec_ns<2>(a, n_items, mixed, opl, threads, lds, s);
-    case 3: return launch_spec_ns<3>(a, n_items, mixed, opl, threads, lds, s);
-    case 4: return launch_spec_ns<4>(a, n_items, mixed, opl, threads, lds, s);
-    case 5: return launch_spec_ns<5>(a, n_items, mixed, opl, threads, lds, s);
-    case 6: return launch_spec_ns<6>(a, n_items, mixed, opl, threads, lds, s);
+    case 1: return launch_spec_ns<1>(a, n_items, mode, opl, threads, lds, s);
+    case 2: return launch_spec_ns<2>(a, n_items, mode, opl, threads, lds, s);
+    case 3: return launch_spec_ns<3>(a, n_items, mode, opl, threads, lds, s);
+    case 4: return launch_spec_ns<4>(a, n_items, mode, opl, threads, lds, s);
+    case 5: return launch_spec_ns<5>(a, n_items, mode, opl, threads, lds, s);
+    case 6: return launch_spec_ns<6>(a, n_items, mode, opl, threads, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -35,7 +35,7 @@ __all__ = [
     "GeneralizedBulkFlow", "fibonacci_directions",
 ]
 
-_PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED}
+_PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED, "fp64_blocked": _lib.FP64_BLOCKED}
 
 
 def _precision(p) -> int:
@@ -43,10 +43,10 @@ def _precision(p) -> int:
         try:
             return _PRECISIONS[p.lower()]
         except KeyError:
-            raise ValueError(f"precision must be 'fp64' or 'mixed', got {p!r}") from None
-    if p in (_lib.FP64, _lib.MIXED):
+            raise ValueError(f"precision must be 'fp64', 'fp64_blocked' or 'mixed', got {p!r}") from None
+    if p in (_lib.FP64, _lib.MIXED, _lib.FP64_BLOCKED):
         return int(p)
-    raise ValueError(f"precision must be 'fp64' or 'mixed', got {p!r}")
+    raise ValueError(f"precision must be 'fp64', 'fp64_blocked' or 'mixed', got {p!r}")
 
 
 # ---------------------------------------------------------------------------

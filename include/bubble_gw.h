@@ -62,13 +62,18 @@ typedef struct bgw_generalized_bulk_flow bgw_generalized_bulk_flow;
 
 /* ---- additions: types ---------------------------------------------------- */
 
-/* Arithmetic of the phase sums.  Fp64: every phase evaluated in double
- * (the drop-in's parity form).  Mixed: collision tests and phase-argument
- * reduction in double, sincos and accumulation in float with compensated
- * summation. */
+/* Arithmetic of the phase sums.
+ *  Fp64:        every phase exp(i w psi) evaluated with its own double sincos
+ *               (the drop-in's parity form).
+ *  Mixed:       collision tests and phase-argument reduction in double; the
+ *               phases of R consecutive time steps factorised as anchor x
+ *               table, sincos, tables, weights and block sums in float, block
+ *               results accumulated in float with Kahan compensation.
+ *  Fp64Blocked: the same block factorisation entirely in double. */
 typedef enum BgwPrecision {
     BgwPrecision_Fp64 = 0,
     BgwPrecision_Mixed = 1,
+    BgwPrecision_Fp64Blocked = 2,
 } BgwPrecision;
 
 /* Device times of the most recent compute call on a handle (HIP events on
