@@ -44,6 +44,14 @@
 
 #include <omp.h>
 
+// Test-only switch: a nonzero ORACLE_PLANT compiles a deliberately wrong
+// variant (1: sign of the time phase, 2: cos-theta step 2/n instead of
+// 2/(n-1), 3: exact ties accepted as first collisions),
+// used by the tests to show that the oracle's checks catch such mistakes.
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -172,7 +180,7 @@ bool build_lattice(const double* interior, size_t n_int, const double* exterior,
   for (size_t a = 0; a < n_int; ++a)
     for (size_t b = a + 1; b < n_int; ++b)
       if (L.dsq(a, b) < 0.0) {
-        g_last_error = "Bubbles Error: Bubble " + index_name(a, n_int) +
+        g_last_error = "Bubble " + index_name(a, n_int) +
                        " is formed inside bubble " + index_name(b, n_int) +
                        " at initial time (overlapping light cones)";
         return false;
@@ -180,7 +188,7 @@ bool build_lattice(const double* interior, size_t n_int, const double* exterior,
   for (size_t a = 0; a < n_int; ++a)
     for (size_t e = 0; e < n_ext; ++e)
       if (L.dsq(a, n_int + e) < 0.0) {
-        g_last_error = "Bubbles Error: Bubble " + index_name(a, n_int) +
+        g_last_error = "Bubble " + index_name(a, n_int) +
                        " is formed inside bubble " + index_name(n_int + e, n_int) +
                        " at initial time (overlapping light cones)";
         return false;
@@ -280,7 +288,11 @@ inline bool check_collision_point(const double* dba, const double* dca, const do
   double nca = lscalar(dca, dca);
   double cv[4];
   for (int k = 0; k < 4; ++k) cv[k] = dba[k] * nca - dca[k] * nba;
+#if ORACLE_PLANT == 3
+  return lscalar(cv, x) >= 0.0;  // planted mistake: ties accepted
+#else
   return lscalar(cv, x) > 0.0;
+#endif
 }
 
 // compute_first_colliding_bubble (:127-206).  Returns -1 for None, otherwise
@@ -416,7 +428,11 @@ void a_integral(const Lattice& L, const Flow& F, size_t a, const double* w_arr, 
   for (size_t ij = 0; ij < n_c * n_phi; ++ij)
     status[ij] = collision_status(first[ij], dtab[ij], delta_ta);
   for (size_t q = 0; q < ns * n_w; ++q) a_plus[q] = a_minus[q] = {0.0, 0.0};
+#if ORACLE_PLANT == 2
+  const double dcos = 2.0 / static_cast<double>(n_c);  // planted mistake: off-by-one step
+#else
   const double dcos = 2.0 / static_cast<double>(n_c - 1);
+#endif
   std::vector<double> bp(ns), bm(ns);
   std::vector<Cx> pf(n_w);
   for (size_t i = 0; i < n_c; ++i) {
@@ -501,7 +517,11 @@ void c_integral_fixed(const Lattice& L, const Flow& F, size_t a, const double* w
       const double weight = (k == 0 || k == nt - 1) ? 0.5 : 1.0;
       for (size_t s = 0; s < ns; ++s) {
         for (size_t q = 0; q < n_w; ++q) {
+#if ORACLE_PLANT == 1
+          Cx phase = cexp_i(-w_arr[q] * (t - z_a));  // planted mistake: phase sign
+#else
           Cx phase = cexp_i(w_arr[q] * (t - z_a));
+#endif
           Cx vp = cscale(cscale(cmul(ap[s * n_w + q], phase), weight), cutoff_weight);
           Cx vm = cscale(cscale(cmul(am[s * n_w + q], phase), weight), cutoff_weight);
           cp_out[s * n_w + q] = {0.0 + vp.re, 0.0 + vp.im};

@@ -1,0 +1,158 @@
+"""C ABI of libbubble_gw_c.so without a GPU: exports, header, host-side validation.
+
+Mirrors the reference's C-ABI tests where they do not need numbers
+(rust/c-bubble-gw/tests/c_api.rs:220-262: the header compiles as C11 and
+C++17) and the error contract of rust/c-bubble-gw/src/lib.rs:104-134,
+386-574.  Nothing here launches a kernel.
+"""
+import ctypes
+import math
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from gw_fopt_amd import _lib
+from gw_fopt_amd import many_bubbles as mb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bubble_gw.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bgw_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert "bgw_generalized_bulk_flow_compute_c_integral" in syms
+    for name in syms:
+        assert hasattr(L, name), name
+    assert set(syms) == set(_lib.SIGNATURES), "ctypes table and header disagree"
+
+
+def test_reference_symbols_present():
+    # the bulk-flow symbols of the reference's cbindgen export list (cbindgen.toml:16-40)
+    ref = ["bgw_version", "bgw_last_error_message", "bgw_generalized_bulk_flow_empty_lattice_new",
+           "bgw_generalized_bulk_flow_free", "bgw_generalized_bulk_flow_set_num_threads",
+           "bgw_generalized_bulk_flow_set_resolution",
+           "bgw_generalized_bulk_flow_set_gradient_scaling_params",
+           "bgw_generalized_bulk_flow_compute_c_integral"]
+    assert set(ref) <= set(declared_symbols())
+
+
+@pytest.mark.parametrize("lang,std,compiler", [("c", "c11", "gcc"), ("c++", "c++17", "g++")])
+def test_header_compiles(tmp_path, lang, std, compiler):
+    if shutil.which(compiler) is None:
+        pytest.skip(f"{compiler} not available")
+    src = tmp_path / ("smoke.c" if lang == "c" else "smoke.cpp")
+    nul = "0" if lang == "c" else "nullptr"
+    src.write_text('#include "bubble_gw.h"\nint main(void) { return bgw_version() == %s; }\n' % nul)
+    subprocess.run([compiler, f"-std={std}", "-fsyntax-only", "-Wall", "-Werror", "-I",
+                    os.path.dirname(HEADER), str(src)], check=True)
+
+
+def test_version_and_clean_error_state():
+    L = _lib.lib()
+    assert L.bgw_version().decode() == "0.1.0"
+    h = ctypes.c_void_p()
+    inter = np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0]])
+    st = L.bgw_generalized_bulk_flow_empty_lattice_new(ctypes.byref(h), _lib.dptr(inter), 2, None, 0)
+    assert st == _lib.OK and h.value
+    assert L.bgw_last_error_message().decode() == ""
+    assert L.bgw_generalized_bulk_flow_set_num_threads(h, 0) == _lib.CALCULATION_ERROR
+    assert L.bgw_last_error_message().decode() == "num_threads must be > 0"
+    assert L.bgw_generalized_bulk_flow_set_num_threads(h, 2) == _lib.OK
+    assert L.bgw_last_error_message().decode() == ""  # cleared on success (lib.rs:121-124)
+    L.bgw_generalized_bulk_flow_free(h)
+    L.bgw_generalized_bulk_flow_free(None)  # null is allowed (lib.rs:424-430)
+
+
+def test_null_out_is_null_pointer_status():
+    L = _lib.lib()
+    inter = np.zeros((1, 4))
+    st = L.bgw_generalized_bulk_flow_empty_lattice_new(None, _lib.dptr(inter), 1, None, 0)
+    assert st == _lib.NULL_POINTER
+    assert L.bgw_last_error_message().decode() == "out must not be null"
+
+
+def test_null_arrays_and_handles():
+    L = _lib.lib()
+    h = ctypes.c_void_p(123)
+    st = L.bgw_generalized_bulk_flow_empty_lattice_new(ctypes.byref(h), None, 3, None, 0)
+    assert st == _lib.CALCULATION_ERROR and h.value is None  # *out set to NULL first (lib.rs:402)
+    assert "bubbles_interior must not be null" in _lib.last_error()
+    assert L.bgw_generalized_bulk_flow_set_resolution(None, 4, 4, True) == _lib.CALCULATION_ERROR
+    assert _lib.last_error() == "bulk_flow must not be null"
+
+
+def test_bubble_inside_bubble_rejected():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    inter = np.array([[0.0, 0.0, 0.0, 0.0], [1.0, 0.5, 0.0, 0.0]])
+    st = L.bgw_generalized_bulk_flow_empty_lattice_new(ctypes.byref(h), _lib.dptr(inter), 2, None, 0)
+    assert st == _lib.CALCULATION_ERROR
+    assert _lib.last_error() == ("Bubble Interior(0) is formed inside bubble Interior(1) at "
+                                 "initial time (overlapping light cones)")
+    ext = np.array([[0.5, 0.2, 0.0, 0.0]])
+    st = L.bgw_generalized_bulk_flow_empty_lattice_new(ctypes.byref(h), _lib.dptr(inter[:1]), 1,
+                                                       _lib.dptr(ext), 1)
+    assert "Exterior(0)" in _lib.last_error()
+
+
+def test_validation_before_any_device_work():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    inter = np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0]])
+    L.bgw_generalized_bulk_flow_empty_lattice_new(ctypes.byref(h), _lib.dptr(inter), 2, None, 0)
+    assert L.bgw_generalized_bulk_flow_set_resolution(h, 1, 5, True) == _lib.CALCULATION_ERROR
+    assert _lib.last_error() == "Invalid resolution: Angular resolutions must be greater than one"
+    w = np.array([0.5, 1.0])
+    out = np.zeros(8)
+    cut = _lib.cutoff_config(None)
+    st = L.bgw_generalized_bulk_flow_compute_c_integral(h, _lib.dptr(w), 2, True, 0.0, 8.0, 1,
+                                                        None, 0, cut, _lib.dptr(out), 4)
+    assert st == _lib.CALCULATION_ERROR
+    assert _lib.last_error() == "Invalid resolution: n_t must be >= 2 for integration"
+    sel = np.array([5], dtype=np.uintp)
+    st = L.bgw_generalized_bulk_flow_compute_c_integral(h, _lib.dptr(w), 2, True, 0.0, 8.0, 40,
+                                                        _lib.sptr(sel), 1, cut, _lib.dptr(out), 4)
+    assert _lib.last_error() == "Index 5 out of bounds for max 2"
+    # empty selection: zeros, no device needed (generalized_bulk_flow.rs:955-957)
+    sel0 = np.zeros(1, dtype=np.uintp)
+    out[:] = 7.0
+    st = L.bgw_generalized_bulk_flow_compute_c_integral(h, _lib.dptr(w), 2, True, 0.0, 8.0, 40,
+                                                        _lib.sptr(sel0), 0, cut, _lib.dptr(out), 4)
+    assert st == _lib.OK and np.all(out[:4] == 0.0), _lib.last_error()
+    st = L.bgw_generalized_bulk_flow_compute_c_integral(h, _lib.dptr(w), 2, True, 0.0, 8.0, 40,
+                                                        _lib.sptr(sel0), 0, cut, _lib.dptr(out), 3)
+    assert _lib.last_error() == "out_len=3 does not match expected length 4"
+    coef = np.array([1.0, 2.0])
+    st = L.bgw_generalized_bulk_flow_set_gradient_scaling_params(h, _lib.dptr(coef), _lib.dptr(coef),
+                                                                 2, 1, math.nan)
+    assert st == _lib.OK
+    st = L.bgw_generalized_bulk_flow_compute_c_integral_ex(h, _lib.dptr(w), 2, True, 0.0, 8.0,
+                                                           40, None, 0, cut, 7, _lib.dptr(out), 8)
+    assert st == _lib.INVALID_ARGUMENT
+    L.bgw_generalized_bulk_flow_free(h)
+
+
+def test_no_device_fails_loudly():
+    """Without a GPU the compute path raises; there is no CPU fallback."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    lb = mb.LatticeBubbles(mb.EmptyLattice(), np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0]]))
+    bf = mb.GeneralizedBulkFlow(lb)
+    bf.set_resolution(4, 5)
+    with pytest.raises(ValueError, match="no HIP device"):
+        bf.compute_c_integral([0.5, 1.0], t_begin=0.0, t_end=8.0, n_t=40)
