@@ -42,7 +42,21 @@ def parse():
     p.add_argument("--modes", default="mixed,fp64_blocked,fp64")
     p.add_argument("--profile-mode", default=None,
                    help="run only this precision for --steps steps (for rocprofv3)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="one device copy of 1 GiB (reads 1 GiB, writes 1 GiB): the reference "
+                        "launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return p.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
 
 
 def algorithmic_flops(wl):
@@ -99,6 +113,9 @@ def cpu_baseline(wl):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding by k-hat is "
                           "described in DESIGN.md but not launched", "n_gpus": args.gpus}))

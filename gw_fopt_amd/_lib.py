@@ -44,8 +44,8 @@ class BgwTimings(ctypes.Structure):
 
 
 class BgwTileConfig(ctypes.Structure):
-    _fields_ = [("mixed_block", ctypes.c_size_t), ("items_per_launch", ctypes.c_size_t),
-                ("persistent_blocks", ctypes.c_size_t)]
+    _fields_ = [("block_steps", ctypes.c_size_t), ("threads", ctypes.c_size_t),
+                ("items_per_chunk", ctypes.c_size_t), ("persistent_blocks", ctypes.c_size_t)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +82,7 @@ SIGNATURES = {
     "bgw_generalized_bulk_flow_spectrum_vjp": (
         _st, [_vp, _dp, _sz, _dp, _sz, _b, _d, _d, _sz, BgwTimeCutoffConfig, _dp, _sz,
               _dp, _dp, _sz]),
+    "bgw_generalized_bulk_flow_set_tiles": (_st, [_vp, ctypes.POINTER(BgwTileConfig)]),
     "bgw_generalized_bulk_flow_last_timings": (_st, [_vp, ctypes.POINTER(BgwTimings)]),
     "bgw_ensemble_new": (_st, [ctypes.POINTER(_vp), _dp, _sp, _dp, _sp, _sz]),
     "bgw_ensemble_free": (None, [_vp]),

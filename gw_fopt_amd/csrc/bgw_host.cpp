@@ -246,7 +246,7 @@ struct Resolution {
 Resolution make_resolution(size_t n_c, size_t n_phi) {
   if (n_c < 2 || n_phi < 2)
     throw CalcError("Invalid resolution: Angular resolutions must be greater than one");
-  if (n_phi > 2048) throw CalcError("Invalid resolution: n_phix > 2048 is not supported");
+  if (n_phi > 1024) throw CalcError("Invalid resolution: n_phix > 1024 is not supported");
   if (n_c > 65535) throw CalcError("Invalid resolution: n_cos_thetax > 65535 is not supported");
   Resolution R;
   R.set = true;
@@ -455,6 +455,8 @@ struct Request {
   // items: per (r, k) group the reference bubbles in summation order
   std::vector<std::vector<int32_t>> bubbles_of_real;  // selected a per realisation
   int mode = 0;  // phase-sum kernel: 0 FP64 direct, 1 FP64 blocked, 2 mixed
+  int tile_threads = 0;  // phase-sum workgroup size (0: automatic)
+  int tile_rb = 16;      // time steps per factorised phase block
   bool want_c = true, want_s = true;
   bool skip_uncollided = true;
 };
@@ -608,7 +610,8 @@ Result run(Engine& E, const Request& q) {
   sa.damping = sc.damping;
   sa.kfac = 1.0 / (6.0 * kPi);
   sa.skip_uncollided = q.skip_uncollided ? 1 : 0;
-  sa.mixed_block = 16;
+  sa.mixed_block = q.tile_rb;
+  sa.threads = q.tile_threads;
   sa.partial = E.d_partial.as<double>();
 
   bgw::RedArgs ra{};
@@ -631,6 +634,8 @@ Result run(Engine& E, const Request& q) {
     sb.n_sets = static_cast<int>(std::min(kMaxSetsPerLaunch, n_sets - s0));
     sb.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
     sb.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
+    sb.time_tables = 0;
+    sb.time_tables = bgw::spectrum_time_tables(sb) ? 1 : 0;
     if (bgw::spectrum_lds_bytes(sb, q.mode) > 160 * 1024)
       throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
     HIPCHECK(bgw::launch_spectrum(sb, static_cast<int>(n_items), q.mode, st));
@@ -721,6 +726,7 @@ struct bgw_generalized_bulk_flow {
   Resolution res;
   Scaling sc;
   size_t num_threads = 0;
+  BgwTileConfig tiles{0, 0, 0, 0};
   Engine engine;
 };
 
@@ -729,11 +735,23 @@ struct bgw_ensemble {
   Resolution res;
   Scaling sc;
   Engine engine;
-  BgwTileConfig tiles{16, 0, 0};
+  BgwTileConfig tiles{0, 0, 0, 0};
   bool tuned = false;
 };
 
 namespace {
+
+void check_tiles(const BgwTileConfig& t) {
+  if (t.block_steps != 0 && t.block_steps != 8 && t.block_steps != 16)
+    throw CalcError("tiles.block_steps must be 0, 8 or 16");
+  if (t.threads != 0 && t.threads != 64 && t.threads != 128 && t.threads != 256)
+    throw CalcError("tiles.threads must be 0, 64, 128 or 256");
+}
+
+void apply_tiles(const BgwTileConfig& t, Request& q) {
+  q.tile_rb = t.block_steps ? static_cast<int>(t.block_steps) : 16;
+  q.tile_threads = static_cast<int>(t.threads);
+}
 
 bgw_generalized_bulk_flow* bulk_flow_handle(bgw_generalized_bulk_flow* h) {
   if (!h) throw CalcError(null_error("bulk_flow"));
@@ -805,6 +823,7 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
   q.bubbles_of_real = {ids};
   q.mode = mode;
   q.want_s = false;
+  apply_tiles(h->tiles, q);
   Result r = run(h->engine, q);
   return r.c;
 }
@@ -1092,6 +1111,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_spectrum(
       q.t_end = {t_end};
       q.bubbles_of_real = {ids};
       q.mode = kernel_mode(precision);
+      apply_tiles(h->tiles, q);
       Result r = run(h->engine, q);
       S = std::move(r.s);
       C = std::move(r.c);
@@ -1118,6 +1138,17 @@ BgwStatus bgw_generalized_bulk_flow_spectrum_vjp(
   (void)grad_coefficients; (void)grad_powers; (void)grad_len;
   set_last_error("spectrum_vjp is not available in this build");
   return BgwStatus_CalculationError;
+}
+
+BgwStatus bgw_generalized_bulk_flow_set_tiles(bgw_generalized_bulk_flow* bulk_flow,
+                                              const BgwTileConfig* tiles) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    BgwTileConfig t{0, 0, 0, 0};
+    if (tiles) t = *tiles;
+    check_tiles(t);
+    h->tiles = t;
+  });
 }
 
 BgwStatus bgw_generalized_bulk_flow_last_timings(bgw_generalized_bulk_flow* bulk_flow,

@@ -49,7 +49,9 @@ struct SpecArgs {
   double damping;
   double kfac;            // 1 / (6 pi)
   int skip_uncollided;    // drop rings whose uncollided part is rounding noise
-  int mixed_block;        // block length of the mixed-precision phase factorisation
+  int mixed_block;        // time steps per factorised phase block (8 or 16)
+  int threads;            // workgroup size (0: automatic); lanes own omegas
+  int time_tables;        // per-item time tables in LDS (spectrum_time_tables)
   double* partial;        // (n_items, 2, n_sets_out, n_w) complex
   // adjoint only
   const double* grad_c;   // (n_groups, 2, n_sets, n_w) complex cotangent of C
@@ -71,6 +73,7 @@ hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s);
 hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s);
 hipError_t launch_reduce(const RedArgs& a, int n_groups, hipStream_t s);
 hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s);
-size_t spectrum_lds_bytes(const SpecArgs& a, int mixed);
+size_t spectrum_lds_bytes(const SpecArgs& a, int mode);
+bool spectrum_time_tables(const SpecArgs& a);
 
 }  // namespace bgw

@@ -42,8 +42,8 @@ namespace bgw {
 namespace {
 
 constexpr int kGeoThreads = 256;
-constexpr int kGeoLdsCand = 256;
-constexpr int kSortCap = 2048;
+constexpr int kGeoLdsCand = 128;
+constexpr int kSortCap = 1024;
 constexpr double kInf = __builtin_inf();
 
 // A candidate in the k-hat frame, as staged in LDS.
@@ -162,14 +162,25 @@ __global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
   double* tau_out = g.tau + static_cast<int64_t>(item_id) * n_dir;
   uint16_t* perm_out = g.perm + static_cast<int64_t>(item_id) * n_dir;
 
+  // Each wavefront takes a compact patch of ph rings x pw azimuths, so its
+  // lanes share first colliders and stop their culled scans together.
+  const int pw = P < 8 ? P : 8;
+  const int ph = 64 / pw;
+  const int patches_per_row = P / pw;
   for (int i0 = 0; i0 < g.n_c; i0 += G) {
     const int gcount = min(G, g.n_c - i0);
     const int nslot = gcount * P;
-    const int nslot_round = ((nslot + blockDim.x - 1) / blockDim.x) * blockDim.x;
-    for (int slot = threadIdx.x; slot < nslot_round; slot += blockDim.x) {
-      const int ring = i0 + slot / P;
-      const int j = slot % P;
-      const bool active = slot < nslot && j < n_phi;
+    const int n_patch = ((gcount + ph - 1) / ph) * patches_per_row;
+    const int nwork_round = ((n_patch * 64 + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    for (int w = threadIdx.x; w < nwork_round; w += blockDim.x) {
+      const int patch = w >> 6, lane = w & 63;
+      const int prow = patch / patches_per_row, pcol = patch % patches_per_row;
+      const int rring = prow * ph + lane / pw;  // ring within the group
+      const int j = pcol * pw + lane % pw;
+      const int ring = i0 + rring;
+      const int slot = rring * P + j;           // sort-buffer position
+      const bool in_group = patch < n_patch && rring < gcount;
+      const bool active = in_group && j < n_phi;
       double x1 = 0.0, x2 = 0.0, x3 = 0.0;
       if (active) {
         const double st = g.sin_t[ring];
@@ -251,7 +262,7 @@ __global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
         }
         if (g.dtab_out) g.dtab_out[static_cast<int64_t>(item_id) * n_dir + dir] = dtab;
       }
-      if (slot < nslot) {
+      if (in_group) {
         sort_key[slot] = tau;
         sort_val[slot] = j;
       }
@@ -336,7 +347,8 @@ __device__ __forceinline__ double cutoff_weight(const SpecArgs& g, double t) {
 
 constexpr int kTimeChunk = 256;
 
-// Per-ring tables in LDS, laid out after each other.
+// Per-ring tables in LDS, laid out after each other, then the per-item time
+// tables (when they fit) and the per-chunk phase inputs.
 struct RingLds {
   double* tau;  // [n_phi] sorted delays
   double* gc;   // [n_phi] w_j cos 2phi_j in sorted order
@@ -345,12 +357,24 @@ struct RingLds {
   double* us;   // [n_phi + 1]
   double* cc;   // [NT][n_phi + 1] prefix sums of gc * tau^p (collided part)
   double* cs;   // [NT][n_phi + 1]
+  double* tb;   // [n_t] K dt w_t cutoff(t) (t - t_a)^3, or null
+  double* tf;   // [NT][n_t] coef_q (t - t_a)^-p_q, or null
   double* psi;  // [kTimeChunk] phase argument per time step
   double* v;    // [kTimeChunk][2 * n_sets] weights V+ / Vx per time step
+  float* vf;    // the same weights in float (mixed form), aliases v
   int stride;   // n_phi + 1
 };
 
-__device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_sets) {
+__host__ __device__ inline size_t ring_lds_doubles(int n_phi, int nt_terms) {
+  return 3 * static_cast<size_t>(n_phi) + 2 * static_cast<size_t>(n_phi + 1) +
+         2 * static_cast<size_t>(nt_terms) * (n_phi + 1);
+}
+
+__host__ __device__ inline size_t time_table_doubles(int n_t, int nt_terms) {
+  return static_cast<size_t>(n_t) * (1 + nt_terms);
+}
+
+__device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool time_tables) {
   RingLds L;
   L.stride = n_phi + 1;
   double* p = smem;
@@ -361,8 +385,15 @@ __device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_sets) {
   L.us = p; p += L.stride;
   L.cc = p; p += static_cast<size_t>(nt_terms) * L.stride;
   L.cs = p; p += static_cast<size_t>(nt_terms) * L.stride;
+  L.tb = nullptr;
+  L.tf = nullptr;
+  if (time_tables) {
+    L.tb = p; p += n_t;
+    L.tf = p; p += static_cast<size_t>(nt_terms) * n_t;
+  }
   L.psi = p; p += kTimeChunk;
   L.v = p;
+  L.vf = reinterpret_cast<float*>(p);
   return L;
 }
 
@@ -432,7 +463,28 @@ __device__ int prepare_ring(const SpecArgs& g, const RingLds& L, int64_t item_id
   return lo;
 }
 
-// V_pm(t) and psi(t) for the time steps [t0, t0 + len) of ring i.
+// K dt w_t cutoff(t) (t - t_a)^3 and coef_q (t - t_a)^-p_q for every time step of
+// an item (ring-independent parts of the weights).  Zero where t <= t_a.
+__device__ void prepare_item_tables(const SpecArgs& g, const RingLds& L, const Item& it,
+                                    const RealInfo& ri) {
+  const int nterms = g.n_sets * g.n_terms;
+  for (int k = threadIdx.x; k < g.n_t; k += blockDim.x) {
+    const double t = ri.t_begin + ri.t_step * static_cast<double>(k);
+    const double dta = t - it.t_a;
+    double base = 0.0;
+    if (dta > 0.0) {
+      const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
+      base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta);
+    }
+    L.tb[k] = base;
+    for (int q = 0; q < nterms; ++q)
+      L.tf[static_cast<size_t>(q) * g.n_t + k] = dta > 0.0 ? g.coef[q] * pow(dta, -g.pows[q]) : 0.0;
+  }
+}
+
+// V_pm(t) and psi(t) for the time steps [t0, t0 + len) of ring i.  FLOAT_V:
+// the weights are stored in float (mixed form).
+template <bool FLOAT_V>
 __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0, int len,
                               const Item& it, const RealInfo& ri) {
   const int n_phi = g.n_phi;
@@ -440,57 +492,68 @@ __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0
   const double c_i = g.ring_c[i];
   const double rf = g.ring_fac[i];
   for (int l = threadIdx.x; l < kTimeChunk; l += blockDim.x) {
-    double* v = L.v + l * 2 * ns;
-    if (l >= len) {
-      for (int s = 0; s < 2 * ns; ++s) v[s] = 0.0;
-      L.psi[l] = 0.0;
-      continue;
-    }
-    const int k = t0 + l;
-    const double t = ri.t_begin + ri.t_step * static_cast<double>(k);
-    const double dta = t - it.t_a;
-    L.psi[l] = (t - it.z_a) - dta * c_i;
-    if (!(dta > 0.0)) {  // compute_b_integral returns zeros (:541-543)
-      for (int s = 0; s < 2 * ns; ++s) v[s] = 0.0;
-      continue;
-    }
-    const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
-    const double base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta) * rf;
-    if (!g.has_damping) {
-      const int cnt = count_le(L.tau, n_phi, dta);
-      for (int s = 0; s < ns; ++s) {
-        double bp = L.uc[cnt], bx = L.us[cnt];
-        for (int kk = 0; kk < g.n_terms; ++kk) {
-          const int q = s * g.n_terms + kk;
-          const double f = g.coef[q] * pow(dta, -g.pows[q]);
-          bp = __builtin_fma(f, L.cc[q * L.stride + cnt], bp);
-          bx = __builtin_fma(f, L.cs[q * L.stride + cnt], bx);
-        }
-        v[2 * s + 0] = base * bp;
-        v[2 * s + 1] = base * bx;
-      }
-    } else {
-      // Direct phi sum with the damping factor exp(-dta (1 - ratio) / width).
-      for (int s = 0; s < ns; ++s) {
-        double bp = 0.0, bx = 0.0;
-        for (int r = 0; r < n_phi; ++r) {
-          const double tau = L.tau[r];
-          double f = 1.0;
-          if (tau <= dta) {
-            const double ratio = tau / dta;
-            f = 0.0;
+    double vv[2 * 8];
+    for (int s = 0; s < 2 * ns; ++s) vv[s] = 0.0;
+    double psi = 0.0;
+    if (l < len) {
+      const int k = t0 + l;
+      const double t = ri.t_begin + ri.t_step * static_cast<double>(k);
+      const double dta = t - it.t_a;
+      psi = (t - it.z_a) - dta * c_i;
+      if (dta > 0.0) {  // compute_b_integral returns zeros for dta <= 0 (:541-543)
+        if (!g.has_damping) {
+          const int cnt = count_le(L.tau, n_phi, dta);
+          double base;
+          if (L.tb) {
+            base = L.tb[k] * rf;
+          } else {
+            const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
+            base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta) * rf;
+          }
+          for (int s = 0; s < ns; ++s) {
+            double bp = L.uc[cnt], bx = L.us[cnt];
             for (int kk = 0; kk < g.n_terms; ++kk) {
               const int q = s * g.n_terms + kk;
-              f += g.coef[q] * pow(ratio, g.pows[q]);
+              const double f = L.tf ? L.tf[static_cast<size_t>(q) * g.n_t + k]
+                                    : g.coef[q] * pow(dta, -g.pows[q]);
+              bp = __builtin_fma(f, L.cc[q * L.stride + cnt], bp);
+              bx = __builtin_fma(f, L.cs[q * L.stride + cnt], bx);
             }
-            f *= exp((-dta) * (1.0 - ratio) / g.damping);
+            vv[2 * s + 0] = base * bp;
+            vv[2 * s + 1] = base * bx;
           }
-          bp = __builtin_fma(L.gc[r], f, bp);
-          bx = __builtin_fma(L.gs[r], f, bx);
+        } else {
+          // Direct phi sum with the damping factor exp(-dta (1 - ratio) / width).
+          const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
+          const double base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta) * rf;
+          for (int s = 0; s < ns; ++s) {
+            double bp = 0.0, bx = 0.0;
+            for (int r = 0; r < n_phi; ++r) {
+              const double tau = L.tau[r];
+              double f = 1.0;
+              if (tau <= dta) {
+                const double ratio = tau / dta;
+                f = 0.0;
+                for (int kk = 0; kk < g.n_terms; ++kk) {
+                  const int q = s * g.n_terms + kk;
+                  f += g.coef[q] * pow(ratio, g.pows[q]);
+                }
+                f *= exp((-dta) * (1.0 - ratio) / g.damping);
+              }
+              bp = __builtin_fma(L.gc[r], f, bp);
+              bx = __builtin_fma(L.gs[r], f, bx);
+            }
+            vv[2 * s + 0] = base * bp;
+            vv[2 * s + 1] = base * bx;
+          }
         }
-        v[2 * s + 0] = base * bp;
-        v[2 * s + 1] = base * bx;
       }
+    }
+    L.psi[l] = psi;
+    if constexpr (FLOAT_V) {
+      for (int s = 0; s < 2 * ns; ++s) L.vf[l * 2 * ns + s] = static_cast<float>(vv[s]);
+    } else {
+      for (int s = 0; s < 2 * ns; ++s) L.v[l * 2 * ns + s] = vv[s];
     }
   }
 }
@@ -514,7 +577,9 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
   const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
   const Item it = g.items[item_id];
   const RealInfo ri = g.real[it.r];
-  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, NS);
+  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, g.time_tables != 0);
+  if (L.tb) prepare_item_tables(g, L, it, ri);
+  __syncthreads();
 
   double om[OPL];
 #pragma unroll
@@ -569,7 +634,7 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
     }
     for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
       const int len = min(kTimeChunk, g.n_t - t0);
-      prepare_times(g, L, i, t0, len, it, ri);
+      prepare_times<MODE == 2>(g, L, i, t0, len, it, ri);
       __syncthreads();
       if constexpr (MODE == 0) {
         for (int l = 0; l < len; ++l) {
@@ -601,11 +666,13 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
               for (int c = 0; c < 4; ++c) bs[q][s][c] = Acc(0);
 #pragma unroll
           for (int r = 0; r < RB; ++r) {
-            const double* v = L.v + (l0 + r) * 2 * NS;  // zero-padded past len
+            const Acc* v;  // zero-padded past len
+            if constexpr (MODE == 2) v = L.vf + (l0 + r) * 2 * NS;
+            else v = L.v + (l0 + r) * 2 * NS;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-              const Acc vp = static_cast<Acc>(v[2 * s]);
-              const Acc vx = static_cast<Acc>(v[2 * s + 1]);
+              const Acc vp = v[2 * s];
+              const Acc vx = v[2 * s + 1];
 #pragma unroll
               for (int q = 0; q < OPL; ++q) {
                 bs[q][s][0] = fma(vp, er[q][r], bs[q][s][0]);
@@ -718,18 +785,21 @@ hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s) {
   return hipGetLastError();
 }
 
-size_t spectrum_lds_bytes(const SpecArgs& a, int /*mixed*/) {
-  const size_t nterms = static_cast<size_t>(a.n_sets) * a.n_terms;
-  const size_t stride = a.n_phi + 1;
-  const size_t doubles = 3 * static_cast<size_t>(a.n_phi) + 2 * stride + 2 * nterms * stride +
-                         kTimeChunk + static_cast<size_t>(kTimeChunk) * 2 * a.n_sets;
+bool spectrum_time_tables(const SpecArgs& a) {
+  return time_table_doubles(a.n_t, a.n_sets * a.n_terms) * sizeof(double) <= 48 * 1024;
+}
+
+size_t spectrum_lds_bytes(const SpecArgs& a, int /*mode*/) {
+  const int nterms = a.n_sets * a.n_terms;
+  size_t doubles = ring_lds_doubles(a.n_phi, nterms) + kTimeChunk +
+                   static_cast<size_t>(kTimeChunk) * 2 * a.n_sets;
+  if (a.time_tables) doubles += time_table_doubles(a.n_t, nterms);
   return doubles * sizeof(double);
 }
 
 namespace {
-template <int NS, int OPL, int MODE>
+template <int NS, int OPL, int MODE, int RB>
 hipError_t launch_spec_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
-  constexpr int RB = 16;
   auto k = spectrum_kernel<NS, OPL, MODE, RB>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -739,36 +809,43 @@ hipError_t launch_spec_t(const SpecArgs& a, int n_items, int threads, size_t lds
   return hipGetLastError();
 }
 
-template <int NS, int MODE>
+template <int NS, int MODE, int RB>
 hipError_t launch_spec_opl(const SpecArgs& a, int n_items, int opl, int threads, size_t lds,
                            hipStream_t s) {
   switch (opl) {
-    case 1: return launch_spec_t<NS, 1, MODE>(a, n_items, threads, lds, s);
-    case 2: return launch_spec_t<NS, 2, MODE>(a, n_items, threads, lds, s);
-    default: return launch_spec_t<NS, 4, MODE>(a, n_items, threads, lds, s);
+    case 1: return launch_spec_t<NS, 1, MODE, RB>(a, n_items, threads, lds, s);
+    case 2: return launch_spec_t<NS, 2, MODE, RB>(a, n_items, threads, lds, s);
+    default: return launch_spec_t<NS, 4, MODE, RB>(a, n_items, threads, lds, s);
   }
 }
 
 template <int NS>
 hipError_t launch_spec_ns(const SpecArgs& a, int n_items, int mode, int opl, int threads,
                           size_t lds, hipStream_t s) {
-  switch (mode) {
-    case 0: return launch_spec_opl<NS, 0>(a, n_items, opl, threads, lds, s);
-    case 1: return launch_spec_opl<NS, 1>(a, n_items, opl, threads, lds, s);
-    default: return launch_spec_opl<NS, 2>(a, n_items, opl, threads, lds, s);
-  }
+  if (mode == 0) return launch_spec_opl<NS, 0, 16>(a, n_items, opl, threads, lds, s);
+  const bool rb8 = a.mixed_block == 8;
+  if (mode == 1)
+    return rb8 ? launch_spec_opl<NS, 1, 8>(a, n_items, opl, threads, lds, s)
+               : launch_spec_opl<NS, 1, 16>(a, n_items, opl, threads, lds, s);
+  return rb8 ? launch_spec_opl<NS, 2, 8>(a, n_items, opl, threads, lds, s)
+             : launch_spec_opl<NS, 2, 16>(a, n_items, opl, threads, lds, s);
 }
 }  // namespace
 
 hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t s) {
   if (n_items == 0) return hipSuccess;
   if (mode < 0 || mode > 2) return hipErrorInvalidValue;
-  // lanes own omegas: 64..256 threads, up to 4 omegas per lane
-  int threads = ((a.n_w + 63) / 64) * 64;
-  if (threads > 256) threads = 256;
+  // Lanes own omegas: threads = 64..256 (a multiple of 64), up to 4 omegas
+  // per lane.  a.threads (0 = automatic) is the tunable choice.
+  int threads = a.threads;
+  if (threads <= 0) {
+    threads = ((a.n_w + 63) / 64) * 64;
+    if (threads > 256) threads = 256;
+  }
+  if (threads < 64 || threads > 256 || threads % 64) return hipErrorInvalidValue;
   const int opl_needed = (a.n_w + threads - 1) / threads;
-  const int opl = opl_needed <= 1 ? 1 : (opl_needed <= 2 ? 2 : 4);
   if (opl_needed > 4) return hipErrorInvalidValue;
+  const int opl = opl_needed <= 1 ? 1 : (opl_needed <= 2 ? 2 : 4);
   const size_t lds = spectrum_lds_bytes(a, mode);
   switch (a.n_sets) {
     case 1: return launch_spec_ns<1>(a, n_items, mode, opl, threads, lds, s);

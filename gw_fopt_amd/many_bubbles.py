@@ -461,6 +461,11 @@ class GeneralizedBulkFlow:
                    _lib.dptr(k), _lib.iptr(first), _lib.dptr(dtab), first.size)
         return first, dtab
 
+    def set_tiles(self, block_steps: int = 0, threads: int = 0) -> None:
+        """Launch shape of the phase-sum kernel (0: automatic)."""
+        t = _lib.BgwTileConfig(int(block_steps), int(threads), 0, 0)
+        self._call(_lib.lib().bgw_generalized_bulk_flow_set_tiles, ctypes.byref(t))
+
     def last_timings(self) -> dict:
         t = _lib.BgwTimings()
         self._call(_lib.lib().bgw_generalized_bulk_flow_last_timings, ctypes.byref(t))

@@ -91,11 +91,12 @@ typedef struct BgwTimings {
 /* A batch of independent nucleation realisations (uneven bubble counts). */
 typedef struct bgw_ensemble bgw_ensemble;
 
-/* Launch shape of the batched form (chosen by bgw_ensemble_autotune). */
+/* Launch shape of the kernels (chosen by bgw_ensemble_autotune; 0 = automatic). */
 typedef struct BgwTileConfig {
-    size_t mixed_block;   /* time steps per factorised phase block: 8 or 16 */
-    size_t items_per_launch; /* work items per persistent chunk (0 = all) */
-    size_t persistent_blocks; /* workgroups of the persistent launch (0 = one per item) */
+    size_t block_steps;       /* time steps per factorised phase block: 8 or 16 */
+    size_t threads;           /* phase-sum workgroup size: 64, 128 or 256 (lanes own omegas) */
+    size_t items_per_chunk;   /* (realisation, k-hat, bubble) items per chunk of the batch */
+    size_t persistent_blocks; /* workgroups of each persistent launch (0: one per item) */
 } BgwTileConfig;
 
 #ifdef __cplusplus
@@ -247,6 +248,10 @@ BgwStatus bgw_generalized_bulk_flow_spectrum_vjp(bgw_generalized_bulk_flow *bulk
                                                  const BgwComplex64 *grad_c, size_t grad_c_len,
                                                  double *grad_coefficients,
                                                  double *grad_powers, size_t grad_len);
+
+/* Launch shape used by this handle's compute calls (NULL: automatic). */
+BgwStatus bgw_generalized_bulk_flow_set_tiles(bgw_generalized_bulk_flow *bulk_flow,
+                                              const BgwTileConfig *tiles);
 
 /* Device timings of the last compute call on this handle. */
 BgwStatus bgw_generalized_bulk_flow_last_timings(bgw_generalized_bulk_flow *bulk_flow,
