@@ -402,7 +402,7 @@ struct Engine {
   // per call
   DevBuf d_items, d_order, d_rot, d_sin_t, d_cos_t, d_cos_p, d_sin_p, d_gcos, d_gsin, d_ring_c,
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
-      d_c, d_s, d_first, d_dtab;
+      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout;
   BgwTimings timings{};
 };
 
@@ -481,19 +481,20 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
   return static_cast<double>(ms);
 }
 
-Result run(Engine& E, const Request& q) {
+// Kernel arguments of one request, after the per-call uploads.
+struct Launch {
+  bgw::GeoArgs ga{};
+  bgw::SpecArgs sa{};
+  bgw::RedArgs ra{};
+  size_t n_items = 0, n_groups = 0;
+};
+
+void setup(Engine& E, const Request& q, Launch& Lc) {
   const Resolution& res = *q.res;
   const Scaling& sc = *q.sc;
   const size_t n_real = E.real.size();
   const size_t n_k = q.k.size();
   const size_t n_sets = sc.n_sets;
-  Result out;
-  out.n_groups = n_real * n_k;
-  if (n_sets == 0 || q.n_w == 0) {
-    out.c.assign(out.n_groups * 2 * n_sets * q.n_w * 2, 0.0);
-    out.s.assign(out.n_groups * n_sets * q.n_w, 0.0);
-    return out;
-  }
   if (q.n_w > 1024) throw CalcError("n_w > 1024 is not supported by the GPU kernels");
   ensure_static(E);
   hipStream_t st = E.dev.stream;
@@ -547,6 +548,8 @@ Result run(Engine& E, const Request& q) {
 
   const size_t n_dir = res.n_c * res.n_phi;
   const size_t nq = n_sets * q.n_w;
+  Lc.n_items = n_items;
+  Lc.n_groups = n_real * n_k;
   upload(E.d_real, real, st);
   upload(E.d_items, items, st);
   upload(E.d_order, order, st);
@@ -566,10 +569,10 @@ Result run(Engine& E, const Request& q) {
   E.d_tau.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(double));
   E.d_perm.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(uint16_t));
   E.d_partial.ensure(std::max<size_t>(1, n_items * 2 * nq) * 2 * sizeof(double));
-  E.d_c.ensure(out.n_groups * 2 * nq * 2 * sizeof(double));
-  E.d_s.ensure(out.n_groups * nq * sizeof(double));
+  E.d_c.ensure(Lc.n_groups * 2 * nq * 2 * sizeof(double));
+  E.d_s.ensure(Lc.n_groups * nq * sizeof(double));
 
-  bgw::GeoArgs ga{};
+  bgw::GeoArgs& ga = Lc.ga;
   ga.cand = E.d_cand.as<CandEntry>();
   ga.inv = E.d_inv.as<int32_t>();
   ga.real = E.d_real.as<RealInfo>();
@@ -586,7 +589,7 @@ Result run(Engine& E, const Request& q) {
   ga.tau = E.d_tau.as<double>();
   ga.perm = E.d_perm.as<uint16_t>();
 
-  bgw::SpecArgs sa{};
+  bgw::SpecArgs& sa = Lc.sa;
   sa.real = ga.real;
   sa.items = ga.items;
   sa.order = E.d_order.as<int32_t>();
@@ -614,7 +617,7 @@ Result run(Engine& E, const Request& q) {
   sa.threads = q.tile_threads;
   sa.partial = E.d_partial.as<double>();
 
-  bgw::RedArgs ra{};
+  bgw::RedArgs& ra = Lc.ra;
   ra.partial = sa.partial;
   ra.group_off = E.d_group_off.as<int32_t>();
   ra.group_r = E.d_group_r.as<int32_t>();
@@ -624,6 +627,28 @@ Result run(Engine& E, const Request& q) {
   ra.n_w = static_cast<int>(q.n_w);
   ra.c_out = E.d_c.as<double>();
   ra.s_out = E.d_s.as<double>();
+}
+
+Result run(Engine& E, const Request& q) {
+  const Resolution& res = *q.res;
+  const Scaling& sc = *q.sc;
+  const size_t n_sets = sc.n_sets;
+  Result out;
+  out.n_groups = E.real.size() * q.k.size();
+  if (n_sets == 0 || q.n_w == 0) {
+    out.c.assign(out.n_groups * 2 * n_sets * q.n_w * 2, 0.0);
+    out.s.assign(out.n_groups * n_sets * q.n_w, 0.0);
+    return out;
+  }
+  Launch Lc;
+  setup(E, q, Lc);
+  hipStream_t st = E.dev.stream;
+  const size_t n_items = Lc.n_items;
+  const size_t n_dir = res.n_c * res.n_phi;
+  const size_t nq = n_sets * q.n_w;
+  const bgw::GeoArgs& ga = Lc.ga;
+  const bgw::SpecArgs& sa = Lc.sa;
+  const bgw::RedArgs& ra = Lc.ra;
 
   HIPCHECK(hipEventRecord(E.dev.ev[0], st));
   HIPCHECK(bgw::launch_geometry(ga, static_cast<int>(n_items), st));
@@ -658,6 +683,59 @@ Result run(Engine& E, const Request& q) {
   E.timings.n_directions = n_items * n_dir;
   E.timings.n_terms = n_items * res.n_c * q.n_t * q.n_w;
   return out;
+}
+
+// Gradient of a loss with respect to the scaling parameters, given the
+// cotangent of C (n_groups, 2, n_sets, n_w) complex.  Recomputes the geometry
+// (first colliders and delays) and runs the adjoint kernel; the per-item
+// gradients are summed on the host in item order.  Returns
+// (2, n_sets, n_terms): d/dcoefficients then d/dpowers.
+std::vector<double> run_adjoint(Engine& E, const Request& q, const double* grad_c) {
+  const Scaling& sc = *q.sc;
+  const size_t n_sets = sc.n_sets, n_terms = sc.n_terms;
+  std::vector<double> grad(2 * n_sets * n_terms, 0.0);
+  if (n_sets == 0 || n_terms == 0 || q.n_w == 0) return grad;
+  Launch Lc;
+  setup(E, q, Lc);
+  hipStream_t st = E.dev.stream;
+  const size_t n_items = Lc.n_items;
+  const size_t nq = n_sets * q.n_w;
+  upload(E.d_gradc, std::vector<double>(grad_c, grad_c + Lc.n_groups * 2 * nq * 2), st);
+  E.d_gradout.ensure(std::max<size_t>(1, n_items * 2 * n_sets * n_terms) * sizeof(double));
+  bgw::SpecArgs sa = Lc.sa;
+  sa.grad_c = E.d_gradc.as<double>();
+  sa.grad_out = E.d_gradout.as<double>();
+  HIPCHECK(hipEventRecord(E.dev.ev[0], st));
+  HIPCHECK(bgw::launch_geometry(Lc.ga, static_cast<int>(n_items), st));
+  HIPCHECK(hipEventRecord(E.dev.ev[1], st));
+  for (size_t s0 = 0; s0 < n_sets;) {
+    bgw::SpecArgs sb = sa;
+    sb.set0 = static_cast<int>(s0);
+    sb.n_sets = static_cast<int>(std::min<size_t>(bgw::kAdjointMaxSets, n_sets - s0));
+    if (bgw::adjoint_lds_bytes(sb, bgw::kAdjointThreads) > 160 * 1024) sb.n_sets = 1;
+    if (bgw::adjoint_lds_bytes(sb, bgw::kAdjointThreads) > 160 * 1024)
+      throw CalcError("resolution too large: the adjoint's per-ring tables exceed the 160 KiB of LDS");
+    sb.coef = E.d_coef.as<double>() + s0 * n_terms;
+    sb.pows = E.d_pows.as<double>() + s0 * n_terms;
+    HIPCHECK(bgw::launch_adjoint(sb, static_cast<int>(n_items), st));
+    s0 += static_cast<size_t>(sb.n_sets);
+  }
+  HIPCHECK(hipEventRecord(E.dev.ev[2], st));
+  std::vector<double> per_item(n_items * 2 * n_sets * n_terms);
+  HIPCHECK(hipMemcpyAsync(per_item.data(), E.d_gradout.p, per_item.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipEventRecord(E.dev.ev[3], st));
+  HIPCHECK(hipStreamSynchronize(st));
+  for (size_t it = 0; it < n_items; ++it)
+    for (size_t e = 0; e < grad.size(); ++e) grad[e] += per_item[it * grad.size() + e];
+  E.timings.geometry_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
+  E.timings.spectrum_ms = elapsed_ms(E.dev.ev[1], E.dev.ev[2]);
+  E.timings.reduce_ms = 0.0;
+  E.timings.total_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[2]);
+  E.timings.n_items = n_items;
+  E.timings.n_directions = n_items * q.res->n_c * q.res->n_phi;
+  E.timings.n_terms = n_items * q.res->n_c * q.n_t * q.n_w;
+  return grad;
 }
 
 // Geometry only, for one item: first colliding bubble and delta_tab.
@@ -795,6 +873,15 @@ void check_time_range(double t_begin, double t_end) {
 }
 
 // compute_c_integral (generalized_bulk_flow.rs:928-984) on the GPU.
+void check_unit_vectors(const double* k_hat, size_t n_k) {
+  for (size_t k = 0; k < n_k; ++k) {
+    const double* kk = k_hat + 3 * k;
+    const double nrm = std::sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
+    if (!(std::fabs(nrm - 1.0) <= 1e-12))
+      throw CalcError("k_hat[" + std::to_string(k) + "] is not a unit vector");
+  }
+}
+
 std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr, size_t n_w,
                               bool has_t_begin, double t_begin, double t_end, size_t n_t,
                               const size_t* selected, size_t n_selected,
@@ -1087,12 +1174,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_spectrum(
     if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
     const double tb = has_t_begin ? t_begin : 0.0;
     const Resolution& res = need_resolution(h->res);
-    for (size_t k = 0; k < n_k; ++k) {
-      const double* kk = k_hat + 3 * k;
-      const double nrm = std::sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
-      if (!(std::fabs(nrm - 1.0) <= 1e-12))
-        throw CalcError("k_hat[" + std::to_string(k) + "] is not a unit vector");
-    }
+    check_unit_vectors(k_hat, n_k);
     std::vector<int32_t> ids;
     for (size_t a = 0; a < h->bubbles.n_int; ++a) ids.push_back(static_cast<int32_t>(a));
     std::vector<double> S(n_k * nq, 0.0), C(n_k * 2 * nq * 2, 0.0);
@@ -1133,11 +1215,47 @@ BgwStatus bgw_generalized_bulk_flow_spectrum_vjp(
     size_t n_k, bool has_t_begin, double t_begin, double t_end, size_t n_t,
     BgwTimeCutoffConfig time_cutoff, const BgwComplex64* grad_c, size_t grad_c_len,
     double* grad_coefficients, double* grad_powers, size_t grad_len) {
-  (void)bulk_flow; (void)w_arr; (void)n_w; (void)k_hat; (void)n_k; (void)has_t_begin;
-  (void)t_begin; (void)t_end; (void)n_t; (void)time_cutoff; (void)grad_c; (void)grad_c_len;
-  (void)grad_coefficients; (void)grad_powers; (void)grad_len;
-  set_last_error("spectrum_vjp is not available in this build");
-  return BgwStatus_CalculationError;
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    slice_from_raw(w_arr, n_w, "w_arr");
+    slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
+    const size_t n_sets = h->sc.n_sets, n_terms = h->sc.n_terms;
+    const size_t nq = n_sets * n_w;
+    if (grad_c_len != n_k * 2 * nq)
+      throw CalcError("grad_c_len=" + std::to_string(grad_c_len) +
+                      " does not match expected length " + std::to_string(n_k * 2 * nq));
+    slice_from_raw(grad_c, grad_c_len, "grad_c");
+    if (grad_len != n_sets * n_terms)
+      throw CalcError("grad_len=" + std::to_string(grad_len) + " does not match expected length " +
+                      std::to_string(n_sets * n_terms));
+    slice_from_raw(grad_coefficients, grad_len, "grad_coefficients");
+    slice_from_raw(grad_powers, grad_len, "grad_powers");
+    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+    const double tb = has_t_begin ? t_begin : 0.0;
+    const Resolution& res = need_resolution(h->res);
+    check_unit_vectors(k_hat, n_k);
+    std::vector<double> g(2 * n_sets * n_terms, 0.0);
+    if (n_k > 0 && h->bubbles.n_int > 0) {
+      check_time_range(tb, t_end);
+      Request q;
+      q.res = &res;
+      q.sc = &h->sc;
+      q.cutoff = make_cutoff(time_cutoff);
+      q.n_w = n_w;
+      q.n_t = n_t;
+      q.identity = false;
+      for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
+      q.omega.assign(w_arr, w_arr + n_w);
+      q.t_begin = {tb};
+      q.t_end = {t_end};
+      std::vector<int32_t> ids(h->bubbles.n_int);
+      std::iota(ids.begin(), ids.end(), 0);
+      q.bubbles_of_real = {ids};
+      g = run_adjoint(h->engine, q, reinterpret_cast<const double*>(grad_c));
+    }
+    std::memcpy(grad_coefficients, g.data(), grad_len * sizeof(double));
+    std::memcpy(grad_powers, g.data() + grad_len, grad_len * sizeof(double));
+  });
 }
 
 BgwStatus bgw_generalized_bulk_flow_set_tiles(bgw_generalized_bulk_flow* bulk_flow,

@@ -55,7 +55,7 @@ struct SpecArgs {
   double* partial;        // (n_items, 2, n_sets_out, n_w) complex
   // adjoint only
   const double* grad_c;   // (n_groups, 2, n_sets, n_w) complex cotangent of C
-  double* grad_out;       // (n_items, 2, n_sets, n_terms): d/dcoef, d/dpow
+  double* grad_out;       // (n_items, 2, n_sets_out, n_terms): d/dcoef, d/dpow
 };
 
 struct RedArgs {
@@ -72,7 +72,11 @@ struct RedArgs {
 hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s);
 hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s);
 hipError_t launch_reduce(const RedArgs& a, int n_groups, hipStream_t s);
+// Adjoint: a.n_sets <= kAdjointMaxSets, workgroups of kAdjointThreads lanes.
+constexpr int kAdjointMaxSets = 2;
+constexpr int kAdjointThreads = 256;
 hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s);
+size_t adjoint_lds_bytes(const SpecArgs& a, int threads);
 size_t spectrum_lds_bytes(const SpecArgs& a, int mode);
 bool spectrum_time_tables(const SpecArgs& a);
 

@@ -357,6 +357,8 @@ struct RingLds {
   double* us;   // [n_phi + 1]
   double* cc;   // [NT][n_phi + 1] prefix sums of gc * tau^p (collided part)
   double* cs;   // [NT][n_phi + 1]
+  double* cl;   // [NT][n_phi + 1] prefix sums of gc * tau^p ln tau (adjoint only)
+  double* cls;  // [NT][n_phi + 1]
   double* tb;   // [n_t] K dt w_t cutoff(t) (t - t_a)^3, or null
   double* tf;   // [NT][n_t] coef_q (t - t_a)^-p_q, or null
   double* psi;  // [kTimeChunk] phase argument per time step
@@ -385,6 +387,8 @@ __device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool ti
   L.us = p; p += L.stride;
   L.cc = p; p += static_cast<size_t>(nt_terms) * L.stride;
   L.cs = p; p += static_cast<size_t>(nt_terms) * L.stride;
+  L.cl = nullptr;
+  L.cls = nullptr;
   L.tb = nullptr;
   L.tf = nullptr;
   if (time_tables) {
@@ -418,6 +422,11 @@ __device__ int prepare_ring(const SpecArgs& g, const RingLds& L, int64_t item_id
       const double pw = fin ? pow(tau, g.pows[q]) : 0.0;
       L.cc[q * L.stride + r + 1] = gcv * pw;
       L.cs[q * L.stride + r + 1] = gsv * pw;
+      if (L.cl) {
+        const double lw = fin ? pw * log(tau) : 0.0;
+        L.cl[q * L.stride + r + 1] = gcv * lw;
+        L.cls[q * L.stride + r + 1] = gsv * lw;
+      }
     }
   }
   if (threadIdx.x == 0) {
@@ -426,18 +435,25 @@ __device__ int prepare_ring(const SpecArgs& g, const RingLds& L, int64_t item_id
     for (int q = 0; q < nterms; ++q) {
       L.cc[q * L.stride] = 0.0;
       L.cs[q * L.stride] = 0.0;
+      if (L.cl) {
+        L.cl[q * L.stride] = 0.0;
+        L.cls[q * L.stride] = 0.0;
+      }
     }
   }
   __syncthreads();
   const int n_waves = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6;
-  const int n_arrays = 2 + 2 * nterms;
+  const int n_prefix = 2 * nterms * (L.cl ? 2 : 1);
+  const int n_arrays = 2 + n_prefix;
   for (int arr = wave; arr < n_arrays; arr += n_waves) {
     if (arr == 0) wave_scan(L.uc, n_phi, true);
     else if (arr == 1) wave_scan(L.us, n_phi, true);
     else {
-      const int q = (arr - 2) >> 1;
-      double* base_arr = ((arr - 2) & 1) ? L.cs : L.cc;
+      const int e = arr - 2;
+      const int q = (e >> 1) % nterms;
+      const bool logs = e >= 2 * nterms;
+      double* base_arr = (e & 1) ? (logs ? L.cls : L.cs) : (logs ? L.cl : L.cc);
       wave_scan(base_arr + q * L.stride + 1, n_phi, false);
     }
   }
@@ -777,6 +793,265 @@ __global__ __launch_bounds__(256) void reduce_kernel(RedArgs g) {
 }
 
 // ---------------------------------------------------------------------------
+// Stage 2 adjoint: gradient of a real loss with respect to the scaling
+// parameters (coefficients and powers)
+// ---------------------------------------------------------------------------
+// C_pm,s(w) = sum_{a,i,t} V_pm,s(t, i) exp(i w psi(t, i)) is linear in the
+// weights V, so for a loss L with cotangent G = dL/dRe C + i dL/dIm C
+//   dL/dtheta = sum_{a,i,t} sum_{pm,s} W_pm,s(t, i) dV_pm,s(t, i)/dtheta,
+//   W_pm,s(t, i) = sum_w Re(conj(G_pm,s(w)) exp(i w psi(t, i))),
+// i.e. the phase sum transposed (over omega instead of over t).  The masks
+// are recomputed by the geometry kernel of the same call.  V depends on the
+// parameters only through the directions already collided at t (tau_j <= dta,
+// a prefix of the sorted ring):
+//   V       = base (U[cnt] + sum_q coef_q dta^-p_q P_q[cnt]),  P_q: prefix of g tau^p_q
+//   dV/dc_q = base dta^-p_q P_q[cnt]
+//   dV/dp_q = base coef_q dta^-p_q (Lg_q[cnt] - ln(dta) P_q[cnt]),  Lg_q: prefix of g tau^p_q ln tau
+// so only time steps with dta >= min tau contribute.  With a damping width the
+// same derivatives are summed directly over the collided directions.
+//
+// Lanes own (block of RB consecutive steps, omega class m mod M); the phase of
+// step t0 + r is exp(i w psi(t0)) E^r with E = exp(i w step (1 - cos th)), by
+// an FP64 recurrence.  The M partial sums of a block are combined by
+// cross-lane shuffles in a fixed order and every lane keeps its own gradient
+// partials, summed in thread order at the end: the result is deterministic.
+namespace {
+
+constexpr int kAdjChunk = 1024;  // time steps of W held in LDS at once
+
+struct AdjLds {
+  RingLds ring;  // tau, gc, gs, uc, us, cc, cs, cl, cls
+  double* om;    // [n_w]
+  double* G;     // [n_w][2 * NS][2] cotangent, index (pm * NS + s)
+  double* E;     // [n_w][2] exp(i w step (1 - cos th)) of the current ring
+  double* W;     // [kAdjChunk][2 * NS]
+  double* acc;   // [2 * nterms][blockDim] per-lane gradient partial sums
+};
+
+__device__ AdjLds carve_adj(double* smem, int n_phi, int nterms, int n_w, int ns) {
+  AdjLds A;
+  RingLds& L = A.ring;
+  L.stride = n_phi + 1;
+  double* p = smem;
+  L.tau = p; p += n_phi;
+  L.gc = p; p += n_phi;
+  L.gs = p; p += n_phi;
+  L.uc = p; p += L.stride;
+  L.us = p; p += L.stride;
+  L.cc = p; p += static_cast<size_t>(nterms) * L.stride;
+  L.cs = p; p += static_cast<size_t>(nterms) * L.stride;
+  L.cl = p; p += static_cast<size_t>(nterms) * L.stride;
+  L.cls = p; p += static_cast<size_t>(nterms) * L.stride;
+  L.tb = nullptr;
+  L.tf = nullptr;
+  L.psi = nullptr;
+  L.v = nullptr;
+  L.vf = nullptr;
+  A.om = p; p += n_w;
+  A.G = p; p += static_cast<size_t>(n_w) * 4 * ns;
+  A.E = p; p += static_cast<size_t>(n_w) * 2;
+  A.W = p; p += static_cast<size_t>(kAdjChunk) * 2 * ns;
+  A.acc = p;
+  return A;
+}
+
+}  // namespace
+
+size_t adjoint_lds_bytes(const SpecArgs& a, int threads) {
+  const size_t nterms = static_cast<size_t>(a.n_sets) * a.n_terms;
+  const size_t stride = static_cast<size_t>(a.n_phi) + 1;
+  const size_t doubles = 3 * static_cast<size_t>(a.n_phi) + 2 * stride + 4 * nterms * stride +
+                         static_cast<size_t>(a.n_w) * (1 + 4 * a.n_sets + 2) +
+                         static_cast<size_t>(kAdjChunk) * 2 * a.n_sets +
+                         2 * nterms * static_cast<size_t>(threads);
+  return doubles * sizeof(double);
+}
+
+template <int NS, int RB>
+__global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
+  extern __shared__ double smem[];
+  const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
+  const Item it = g.items[item_id];
+  const RealInfo ri = g.real[it.r];
+  const int nterms = NS * g.n_terms;
+  const int bd = blockDim.x;
+  const int tid = threadIdx.x;
+  const AdjLds A = carve_adj(smem, g.n_phi, nterms, g.n_w, NS);
+  const RingLds& L = A.ring;
+
+  // cotangent of this item's (realisation, k-hat) group, sets [set0, set0 + NS)
+  const int64_t nq = static_cast<int64_t>(g.n_sets_out) * g.n_w;
+  const double* gsrc = g.grad_c + static_cast<int64_t>(it.group) * 2 * nq * 2;
+  for (int j = tid; j < g.n_w; j += bd) {
+    A.om[j] = g.omega[ri.w_off + j];
+    for (int pm = 0; pm < 2; ++pm)
+      for (int s = 0; s < NS; ++s) {
+        const int64_t src = (pm * nq + static_cast<int64_t>(g.set0 + s) * g.n_w + j) * 2;
+        double* dst = A.G + (static_cast<int64_t>(j) * 2 * NS + pm * NS + s) * 2;
+        dst[0] = gsrc[src];
+        dst[1] = gsrc[src + 1];
+      }
+  }
+  for (int e = 0; e < 2 * nterms; ++e) A.acc[e * bd + tid] = 0.0;
+  __syncthreads();
+
+  for (int i = 0; i < g.n_c; ++i) {
+    const double rf = g.ring_fac[i];
+    if (rf == 0.0) continue;  // sin^2 theta = 0: B = 0 exactly
+    bool unused;
+    (void)prepare_ring(g, L, item_id, i, it.t_a, ri, &unused);
+    const double tau_min = L.tau[0];
+    if (!(tau_min < kInf)) {  // ring never collides: no parameter dependence
+      __syncthreads();
+      continue;
+    }
+    int lo = 0, hi = g.n_t;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const double t = ri.t_begin + ri.t_step * static_cast<double>(mid);
+      const double dta = t - it.t_a;
+      if (dta > 0.0 && dta >= tau_min) hi = mid;
+      else lo = mid + 1;
+    }
+    const int t_lo = lo;
+    if (t_lo >= g.n_t) {
+      __syncthreads();
+      continue;
+    }
+    const double c_i = g.ring_c[i];
+    const double delta = ri.t_step * (1.0 - c_i);
+    for (int j = tid; j < g.n_w; j += bd) {
+      double sn, cs;
+      sincos_f64(A.om[j] * delta, &sn, &cs);
+      A.E[2 * j] = cs;
+      A.E[2 * j + 1] = sn;
+    }
+    __syncthreads();
+
+    for (int c0 = t_lo; c0 < g.n_t; c0 += kAdjChunk) {
+      const int len = min(kAdjChunk, g.n_t - c0);
+      const int n_blk = (len + RB - 1) / RB;
+      int mshift = 6;  // M = 2^mshift omega classes per block, M <= 64
+      while (mshift > 0 && (n_blk << mshift) > bd) --mshift;
+      const int M = 1 << mshift;
+      const int n_units = n_blk << mshift;
+      const int n_iter = (n_units + bd - 1) / bd;
+      for (int itn = 0; itn < n_iter; ++itn) {
+        const int u = itn * bd + tid;
+        const int blk = u >> mshift;
+        const int m = u & (M - 1);
+        const bool live = u < n_units;
+        double w[RB][2 * NS];
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+          for (int c = 0; c < 2 * NS; ++c) w[r][c] = 0.0;
+        if (live) {
+          const int k0 = c0 + blk * RB;
+          const double t0 = ri.t_begin + ri.t_step * static_cast<double>(k0);
+          const double psi0 = (t0 - it.z_a) - (t0 - it.t_a) * c_i;
+          for (int j = m; j < g.n_w; j += M) {
+            double zi, zr;
+            sincos_f64(A.om[j] * psi0, &zi, &zr);
+            const double er = A.E[2 * j], ei = A.E[2 * j + 1];
+            double gr[2 * NS], gi[2 * NS];
+#pragma unroll
+            for (int c = 0; c < 2 * NS; ++c) {
+              gr[c] = A.G[(j * 2 * NS + c) * 2];
+              gi[c] = A.G[(j * 2 * NS + c) * 2 + 1];
+            }
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+#pragma unroll
+              for (int c = 0; c < 2 * NS; ++c)
+                w[r][c] = __builtin_fma(gr[c], zr, __builtin_fma(gi[c], zi, w[r][c]));
+              const double nr = zr * er - zi * ei;
+              const double ni = zr * ei + zi * er;
+              zr = nr;
+              zi = ni;
+            }
+          }
+        }
+        // sum the M omega classes of each block (adjacent lanes of one wave)
+        for (int off = M >> 1; off > 0; off >>= 1) {
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int c = 0; c < 2 * NS; ++c) w[r][c] += __shfl_xor(w[r][c], off, 64);
+        }
+        if (live && m == 0) {
+          const int l0 = blk * RB;
+#pragma unroll
+          for (int r = 0; r < RB; ++r) {
+            if (l0 + r < len) {
+#pragma unroll
+              for (int c = 0; c < 2 * NS; ++c) A.W[(l0 + r) * 2 * NS + c] = w[r][c];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // contraction with dV/dtheta
+      for (int l = tid; l < len; l += bd) {
+        const int k = c0 + l;
+        const double t = ri.t_begin + ri.t_step * static_cast<double>(k);
+        const double dta = t - it.t_a;  // >= tau_min > 0
+        const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
+        const double base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta) * rf;
+        const double* Wt = A.W + l * 2 * NS;
+        const int cnt = count_le(L.tau, g.n_phi, dta);
+        if (!g.has_damping) {
+          const double ldta = log(dta);
+          for (int s = 0; s < NS; ++s) {
+            const double wp = Wt[s], wx = Wt[NS + s];
+            for (int kk = 0; kk < g.n_terms; ++kk) {
+              const int q = s * g.n_terms + kk;
+              const double dmp = pow(dta, -g.pows[q]);
+              const double pp = L.cc[q * L.stride + cnt], px = L.cs[q * L.stride + cnt];
+              const double lp = L.cl[q * L.stride + cnt], lx = L.cls[q * L.stride + cnt];
+              A.acc[(2 * q) * bd + tid] += base * dmp * (wp * pp + wx * px);
+              A.acc[(2 * q + 1) * bd + tid] +=
+                  base * g.coef[q] * dmp * (wp * (lp - ldta * pp) + wx * (lx - ldta * px));
+            }
+          }
+        } else {
+          for (int s = 0; s < NS; ++s) {
+            const double wp = Wt[s], wx = Wt[NS + s];
+            for (int kk = 0; kk < g.n_terms; ++kk) {
+              const int q = s * g.n_terms + kk;
+              double dcp = 0.0, dcx = 0.0, dpp = 0.0, dpx = 0.0;
+              for (int r = 0; r < cnt; ++r) {
+                const double ratio = L.tau[r] / dta;
+                const double d = exp((-dta) * (1.0 - ratio) / g.damping);
+                const double rp = pow(ratio, g.pows[q]) * d;
+                const double rl = rp * log(ratio);
+                dcp = __builtin_fma(L.gc[r], rp, dcp);
+                dcx = __builtin_fma(L.gs[r], rp, dcx);
+                dpp = __builtin_fma(L.gc[r], rl, dpp);
+                dpx = __builtin_fma(L.gs[r], rl, dpx);
+              }
+              A.acc[(2 * q) * bd + tid] += base * (wp * dcp + wx * dcx);
+              A.acc[(2 * q + 1) * bd + tid] += base * g.coef[q] * (wp * dpp + wx * dpx);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // grad_out[item][which][set][term], which 0: d/dcoef, 1: d/dpow
+  for (int e = tid; e < 2 * nterms; e += bd) {
+    const int q = e >> 1, which = e & 1;
+    const int s = q / g.n_terms, kk = q % g.n_terms;
+    double sum = 0.0;
+    for (int l = 0; l < bd; ++l) sum += A.acc[e * bd + l];
+    const int64_t o = ((static_cast<int64_t>(item_id) * 2 + which) * g.n_sets_out + g.set0 + s) *
+                          g.n_terms + kk;
+    g.grad_out[o] = sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s) {
@@ -854,6 +1129,31 @@ hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t
     case 4: return launch_spec_ns<4>(a, n_items, mode, opl, threads, lds, s);
     case 5: return launch_spec_ns<5>(a, n_items, mode, opl, threads, lds, s);
     case 6: return launch_spec_ns<6>(a, n_items, mode, opl, threads, lds, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+namespace {
+template <int NS, int RB>
+hipError_t launch_adj_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
+  auto k = adjoint_kernel<NS, RB>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(lds));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(n_items), dim3(threads), lds, s, a);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s) {
+  if (n_items == 0) return hipSuccess;
+  const int threads = kAdjointThreads;
+  const size_t lds = adjoint_lds_bytes(a, threads);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  switch (a.n_sets) {
+    case 1: return launch_adj_t<1, 16>(a, n_items, threads, lds, s);
+    case 2: return launch_adj_t<2, 8>(a, n_items, threads, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -449,6 +449,30 @@ class GeneralizedBulkFlow:
                    0 if C is None else C.size)
         return (S, M, C) if return_c else (S, M)
 
+    def spectrum_vjp(self, w_arr, k_hat, grad_c, *, t_begin=None, t_end, n_t, time_cutoff=None):
+        """Gradient of a real loss L with respect to the scaling parameters.
+
+        ``grad_c`` is dL/dRe C + i dL/dIm C for the C of :meth:`compute_spectrum`
+        (n_k, 2, n_sets, n_w) complex.  Returns ``(dL/dcoefficients,
+        dL/dpowers)``, each (n_sets, n_terms).  The hand-written FP64 adjoint
+        kernel recomputes the collision masks.
+        """
+        w = np.ascontiguousarray(np.asarray(w_arr, dtype=np.float64).reshape(-1))
+        k = np.ascontiguousarray(np.asarray(k_hat, dtype=np.float64).reshape(-1, 3))
+        self._push_params()
+        n_sets, n_terms = self._coef.shape
+        g = np.ascontiguousarray(np.asarray(grad_c, dtype=np.complex128))
+        if g.shape != (len(k), 2, n_sets, len(w)):
+            raise ValueError(f"grad_c has shape {g.shape}, expected {(len(k), 2, n_sets, len(w))}")
+        dc = np.empty((n_sets, n_terms))
+        dp = np.empty((n_sets, n_terms))
+        self._call(_lib.lib().bgw_generalized_bulk_flow_spectrum_vjp, _lib.dptr(w), len(w),
+                   _lib.dptr(k), len(k), t_begin is not None,
+                   0.0 if t_begin is None else float(t_begin), float(t_end), int(n_t),
+                   _lib.cutoff_config(time_cutoff), g.ctypes.data_as(_lib._dp), g.size,
+                   _lib.dptr(dc), _lib.dptr(dp), dc.size)
+        return dc, dp
+
     def first_colliding_in_frame(self, a_idx: int, k_hat=None):
         """(first_bubble int32, delta_tab float64), both (n_c, n_phi), as computed by the
         GPU geometry kernel in the frame where ``k_hat`` -> z (None: reference frame)."""

@@ -143,6 +143,39 @@ def test_linear_in_coefficients():
     np.testing.assert_allclose(c[2], c[0] + 0.37 * (c[1] - c[0]), rtol=0, atol=1e-13 * np.abs(c[1]).max())
 
 
+def test_tangent_coef_is_the_linear_difference():
+    """C is linear in each coefficient: dC/da_k = C(a_k = 1) - C(a_k = 0), other terms fixed."""
+    inter, ext = async_bubbles(13)
+    w = np.geomspace(0.5, 50, 5)
+    kw = dict(t_end=0.8, n_t=30, k_hat=[0.6, 0.0, 0.8])
+    pows = [[5.0, 3.5]]
+    T = oracle.compute_c_integral(inter, ext, 8, 12, [[0.7, 0.2]], pows, w, derivative=("pow", 0),
+                                  **kw)
+    d1 = oracle.compute_c_integral(inter, ext, 8, 12, [[0.7, 0.2]], pows, w,
+                                   derivative=("coef", 1), **kw)
+    hi = oracle.compute_c_integral(inter, ext, 8, 12, [[0.7, 1.0]], pows, w, **kw)
+    lo = oracle.compute_c_integral(inter, ext, 8, 12, [[0.7, 0.0]], pows, w, **kw)
+    np.testing.assert_allclose(d1, hi - lo, rtol=0, atol=1e-12 * np.abs(hi - lo).max())
+    assert np.abs(T).max() > 0 and np.abs(d1).max() > 0
+
+
+@pytest.mark.parametrize("damping", [None, 0.3])
+def test_tangent_pow_matches_central_differences(damping):
+    """dC/dp_k (forward mode, f -> a ratio^p ln ratio) against (C(p+h) - C(p-h)) / 2h."""
+    inter, ext = async_bubbles(14)
+    w = np.geomspace(0.5, 50, 5)
+    kw = dict(t_end=0.8, n_t=30, damping=damping, k_hat=[0.0, 0.6, 0.8])
+    coef = [[0.7, 0.4], [1.1, 0.0]]
+    T = oracle.compute_c_integral(inter, ext, 8, 12, coef, [[5.0, 3.5], [4.2, 3.0]], w,
+                                  derivative=("pow", 1), **kw)
+    h = 1e-5
+    up = oracle.compute_c_integral(inter, ext, 8, 12, coef, [[5.0, 3.5 + h], [4.2, 3.0 + h]], w, **kw)
+    dn = oracle.compute_c_integral(inter, ext, 8, 12, coef, [[5.0, 3.5 - h], [4.2, 3.0 - h]], w, **kw)
+    fd = (up - dn) / (2 * h)
+    np.testing.assert_allclose(T[:, 0], fd[:, 0], rtol=0, atol=1e-7 * np.abs(fd[:, 0]).max())
+    assert np.abs(T[:, 1]).max() == 0.0  # coefficient 0: the set does not depend on p
+
+
 def test_sets_equal_single_runs():
     inter, ext = async_bubbles(11)
     w = np.geomspace(0.5, 50, 5)
