@@ -402,7 +402,7 @@ struct Engine {
   // per call
   DevBuf d_items, d_order, d_rot, d_sin_t, d_cos_t, d_cos_p, d_sin_p, d_gcos, d_gsin, d_ring_c,
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
-      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout;
+      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
   BgwTimings timings{};
 };
 
@@ -736,6 +736,64 @@ std::vector<double> run_adjoint(Engine& E, const Request& q, const double* grad_
   E.timings.n_directions = n_items * q.res->n_c * q.res->n_phi;
   E.timings.n_terms = n_items * q.res->n_c * q.n_t * q.n_w;
   return grad;
+}
+
+// Time-resolved integrand, (2, n_sets, n_t, n_w) complex summed over the
+// request's bubbles in order.  Items run in chunks whose per-item slabs fit
+// kIntegrandScratch bytes; each chunk is added to the total in item order.
+constexpr size_t kIntegrandScratch = size_t(1) << 30;
+
+std::vector<double> run_integrand(Engine& E, const Request& q) {
+  const size_t n_sets = q.sc->n_sets;
+  const size_t n_elem = 2 * n_sets * q.n_t * q.n_w * 2;  // doubles per item
+  std::vector<double> out(n_elem, 0.0);
+  if (n_sets == 0 || q.n_w == 0) return out;
+  Launch Lc;
+  setup(E, q, Lc);
+  hipStream_t st = E.dev.stream;
+  const size_t n_items = Lc.n_items;
+  const size_t chunk =
+      std::max<size_t>(1, std::min(n_items, kIntegrandScratch / (n_elem * sizeof(double))));
+  std::vector<int32_t> iota(n_items);
+  std::iota(iota.begin(), iota.end(), 0);
+  upload(E.d_iota, iota, st);
+  E.d_partial.ensure(chunk * n_elem * sizeof(double));
+  E.d_acc.ensure(n_elem * sizeof(double));
+  HIPCHECK(hipEventRecord(E.dev.ev[0], st));
+  HIPCHECK(bgw::launch_geometry(Lc.ga, static_cast<int>(n_items), st));
+  HIPCHECK(hipEventRecord(E.dev.ev[1], st));
+  for (size_t c0 = 0; c0 < n_items; c0 += chunk) {
+    const size_t nc = std::min(chunk, n_items - c0);
+    for (size_t s0 = 0; s0 < n_sets; s0 += kMaxSetsPerLaunch) {
+      bgw::SpecArgs sb = Lc.sa;
+      sb.order = E.d_iota.as<int32_t>() + c0;
+      sb.partial = E.d_partial.as<double>();
+      sb.set0 = static_cast<int>(s0);
+      sb.n_sets = static_cast<int>(std::min(kMaxSetsPerLaunch, n_sets - s0));
+      sb.coef = E.d_coef.as<double>() + s0 * q.sc->n_terms;
+      sb.pows = E.d_pows.as<double>() + s0 * q.sc->n_terms;
+      sb.time_tables = 0;
+      if (bgw::spectrum_lds_bytes(sb, 0) > 160 * 1024)
+        throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+      HIPCHECK(bgw::launch_integrand(sb, static_cast<int>(nc), st));
+    }
+    HIPCHECK(bgw::launch_accumulate(E.d_partial.as<double>(), static_cast<int>(nc),
+                                    static_cast<int64_t>(n_elem), c0 == 0 ? 1 : 0,
+                                    E.d_acc.as<double>(), st));
+  }
+  HIPCHECK(hipEventRecord(E.dev.ev[2], st));
+  HIPCHECK(hipMemcpyAsync(out.data(), E.d_acc.p, n_elem * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
+  HIPCHECK(hipEventRecord(E.dev.ev[3], st));
+  HIPCHECK(hipStreamSynchronize(st));
+  E.timings.geometry_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
+  E.timings.spectrum_ms = elapsed_ms(E.dev.ev[1], E.dev.ev[2]);
+  E.timings.reduce_ms = 0.0;
+  E.timings.total_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[2]);
+  E.timings.n_items = n_items;
+  E.timings.n_directions = n_items * q.res->n_c * q.res->n_phi;
+  E.timings.n_terms = n_items * q.res->n_c * q.n_t * q.n_w;
+  return out;
 }
 
 // Geometry only, for one item: first colliding bubble and delta_tab.
@@ -1140,10 +1198,53 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integrand(
     bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
     double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
     size_t n_selected_bubbles, BgwComplex64* out, size_t out_len) {
-  (void)bulk_flow; (void)w_arr; (void)n_w; (void)has_t_begin; (void)t_begin; (void)t_end;
-  (void)n_t; (void)selected_bubbles; (void)n_selected_bubbles; (void)out; (void)out_len;
-  set_last_error("compute_c_integrand is not available in this build");
-  return BgwStatus_CalculationError;
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    slice_from_raw(w_arr, n_w, "w_arr");
+    const size_t n_sets = h->sc.n_sets;
+    // compute_c_integrand (generalized_bulk_flow.rs:875-920)
+    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2");
+    bool empty = false;
+    std::vector<int32_t> ids = selected_ids(h, selected_bubbles, n_selected_bubbles, &empty);
+    const size_t expect = 2 * n_sets * n_w * n_t;
+    if (out_len != expect)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(expect));
+    slice_from_raw(out, out_len, "out");
+    std::vector<double> res_t(expect * 2, 0.0);  // (2, n_sets, n_t, n_w)
+    const double tb = has_t_begin ? t_begin : 0.0;
+    // a bubble nucleated at or after t_end contributes zeros before the time
+    // range is checked (:681-692)
+    bool live = false;
+    for (int32_t a : ids) live = live || h->bubbles.row(static_cast<size_t>(a))[0] < t_end;
+    if (live) {
+      check_time_range(tb, t_end);
+      const Resolution& res = need_resolution(h->res);
+      Request q;
+      q.res = &res;
+      q.sc = &h->sc;
+      q.cutoff = Cutoff{};
+      q.n_w = n_w;
+      q.n_t = n_t;
+      q.identity = true;
+      q.k.push_back({0.0, 0.0, 1.0});
+      q.omega.assign(w_arr, w_arr + n_w);
+      q.t_begin = {tb};
+      q.t_end = {t_end};
+      q.bubbles_of_real = {ids};
+      q.skip_uncollided = false;
+      res_t = run_integrand(h->engine, q);
+    }
+    double* o = reinterpret_cast<double*>(out);
+    for (size_t ps = 0; ps < 2 * n_sets; ++ps)
+      for (size_t w = 0; w < n_w; ++w)
+        for (size_t k = 0; k < n_t; ++k) {
+          const size_t src = ((ps * n_t + k) * n_w + w) * 2;
+          const size_t dst = ((ps * n_w + w) * n_t + k) * 2;
+          o[dst] = res_t[src];
+          o[dst + 1] = res_t[src + 1];
+        }
+  });
 }
 
 BgwStatus bgw_generalized_bulk_flow_compute_spectrum(

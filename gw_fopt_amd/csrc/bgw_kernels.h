@@ -77,6 +77,12 @@ constexpr int kAdjointMaxSets = 2;
 constexpr int kAdjointThreads = 256;
 hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s);
 size_t adjoint_lds_bytes(const SpecArgs& a, int threads);
+// Time-resolved integrand: items [0, n_items) of a.order write
+// a.partial[block][pm][set][t][w] (complex, a.n_sets_out sets); FP64.
+hipError_t launch_integrand(const SpecArgs& a, int n_items, hipStream_t s);
+// acc[e] = (first ? 0 : acc[e]) + sum_{it < n_chunk_items} partial[it][e], in order.
+hipError_t launch_accumulate(const double* partial, int n_chunk_items, int64_t n_elem, int first,
+                             double* acc, hipStream_t s);
 size_t spectrum_lds_bytes(const SpecArgs& a, int mode);
 bool spectrum_time_tables(const SpecArgs& a);
 

@@ -793,6 +793,90 @@ __global__ __launch_bounds__(256) void reduce_kernel(RedArgs g) {
 }
 
 // ---------------------------------------------------------------------------
+// Stage 2, time-resolved: the integrand of the time integral
+// ---------------------------------------------------------------------------
+// compute_c_integrand_fixed_bubble (generalized_bulk_flow.rs:666-767): per time
+// step t the sum over rings of V_pm(t, i) exp(i w psi(t, i)) (no time cutoff,
+// zero for t <= t_a).  Lanes own omegas; rings run in order and each lane adds
+// its ring contribution into its own entries of the item's slab
+//   partial[item][pm][s][t][w]   (complex, sets of the whole request)
+// so no two lanes touch the same entry.  FP64 sincos, as the FP64 form.
+template <int NS, int OPL>
+__global__ __launch_bounds__(256) void integrand_kernel(SpecArgs g) {
+  extern __shared__ double smem[];
+  const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
+  const Item it = g.items[item_id];
+  const RealInfo ri = g.real[it.r];
+  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, false);
+  const int64_t slab = static_cast<int64_t>(g.n_t) * g.n_w;  // one (pm, s)
+  double* out = g.partial + static_cast<int64_t>(blockIdx.x) * 2 * g.n_sets_out * slab * 2;
+  double om[OPL];
+#pragma unroll
+  for (int q = 0; q < OPL; ++q) {
+    const int wi = threadIdx.x + q * blockDim.x;
+    om[q] = wi < g.n_w ? g.omega[ri.w_off + wi] : 0.0;
+    if (wi >= g.n_w) continue;
+    for (int pm = 0; pm < 2; ++pm)
+      for (int s = 0; s < NS; ++s) {
+        double* o = out + ((pm * g.n_sets_out + g.set0 + s) * slab + wi) * 2;
+        for (int k = 0; k < g.n_t; ++k) {
+          o[static_cast<int64_t>(k) * g.n_w * 2] = 0.0;
+          o[static_cast<int64_t>(k) * g.n_w * 2 + 1] = 0.0;
+        }
+      }
+  }
+  for (int i = 0; i < g.n_c; ++i) {
+    if (g.ring_fac[i] == 0.0) continue;
+    bool skip;
+    const int t_lo = prepare_ring(g, L, item_id, i, it.t_a, ri, &skip);
+    if (skip || t_lo >= g.n_t) {
+      __syncthreads();
+      continue;
+    }
+    for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
+      const int len = min(kTimeChunk, g.n_t - t0);
+      prepare_times<false>(g, L, i, t0, len, it, ri);
+      __syncthreads();
+      for (int l = 0; l < len; ++l) {
+        const double psi = L.psi[l];
+        const double* v = L.v + l * 2 * NS;
+        const int64_t kofs = static_cast<int64_t>(t0 + l) * g.n_w;
+#pragma unroll
+        for (int q = 0; q < OPL; ++q) {
+          const int wi = threadIdx.x + q * blockDim.x;
+          if (wi >= g.n_w) continue;
+          double sn, cs;
+          sincos_f64(om[q] * psi, &sn, &cs);
+#pragma unroll
+          for (int s = 0; s < NS; ++s) {
+            double* op = out + ((g.set0 + s) * slab + kofs + wi) * 2;
+            double* ox = out + ((g.n_sets_out + g.set0 + s) * slab + kofs + wi) * 2;
+            const double vp = v[2 * s], vx = v[2 * s + 1];
+            op[0] = __builtin_fma(vp, cs, op[0]);
+            op[1] = __builtin_fma(vp, sn, op[1]);
+            ox[0] = __builtin_fma(vx, cs, ox[0]);
+            ox[1] = __builtin_fma(vx, sn, ox[1]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// acc[e] = (first ? 0 : acc[e]) + sum over the chunk's items, in item order.
+__global__ __launch_bounds__(256) void accumulate_kernel(const double* __restrict__ partial,
+                                                         int n_chunk_items, int64_t n_elem,
+                                                         int first, double* __restrict__ acc) {
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n_elem;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double sum = first ? 0.0 : acc[e];
+    for (int it = 0; it < n_chunk_items; ++it) sum += partial[static_cast<int64_t>(it) * n_elem + e];
+    acc[e] = sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stage 2 adjoint: gradient of a real loss with respect to the scaling
 // parameters (coefficients and powers)
 // ---------------------------------------------------------------------------
@@ -1156,6 +1240,60 @@ hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s) {
     case 2: return launch_adj_t<2, 8>(a, n_items, threads, lds, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+namespace {
+template <int NS, int OPL>
+hipError_t launch_int_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
+  auto k = integrand_kernel<NS, OPL>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(lds));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(n_items), dim3(threads), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_int_ns(const SpecArgs& a, int n_items, int opl, int threads, size_t lds,
+                         hipStream_t s) {
+  switch (opl) {
+    case 1: return launch_int_t<NS, 1>(a, n_items, threads, lds, s);
+    case 2: return launch_int_t<NS, 2>(a, n_items, threads, lds, s);
+    default: return launch_int_t<NS, 4>(a, n_items, threads, lds, s);
+  }
+}
+}  // namespace
+
+hipError_t launch_integrand(const SpecArgs& a, int n_items, hipStream_t s) {
+  if (n_items == 0) return hipSuccess;
+  int threads = ((a.n_w + 63) / 64) * 64;
+  if (threads > 256) threads = 256;
+  const int opl_needed = (a.n_w + threads - 1) / threads;
+  if (opl_needed > 4) return hipErrorInvalidValue;
+  const int opl = opl_needed <= 1 ? 1 : (opl_needed <= 2 ? 2 : 4);
+  SpecArgs b = a;
+  b.time_tables = 0;
+  const size_t lds = spectrum_lds_bytes(b, 0);
+  switch (a.n_sets) {
+    case 1: return launch_int_ns<1>(b, n_items, opl, threads, lds, s);
+    case 2: return launch_int_ns<2>(b, n_items, opl, threads, lds, s);
+    case 3: return launch_int_ns<3>(b, n_items, opl, threads, lds, s);
+    case 4: return launch_int_ns<4>(b, n_items, opl, threads, lds, s);
+    case 5: return launch_int_ns<5>(b, n_items, opl, threads, lds, s);
+    case 6: return launch_int_ns<6>(b, n_items, opl, threads, lds, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_accumulate(const double* partial, int n_chunk_items, int64_t n_elem, int first,
+                             double* acc, hipStream_t s) {
+  if (n_elem == 0) return hipSuccess;
+  int64_t blocks = (n_elem + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(accumulate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                     partial, n_chunk_items, n_elem, first, acc);
+  return hipGetLastError();
 }
 
 hipError_t launch_reduce(const RedArgs& a, int n_groups, hipStream_t s) {

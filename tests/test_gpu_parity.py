@@ -186,3 +186,43 @@ def test_identity_frame_equals_reference_frame():
     _, _, Ck = bf.compute_spectrum(wl.w_arr, [[0.0, 0.0, 1.0]], t_end=wl.t_end, n_t=80,
                                    return_c=True)
     assert np.array_equal(C, Ck[0])
+
+
+def integrand_rel(a, b):
+    """Worst omega: max over t of |a - b| over max over t of |b| (per omega bin)."""
+    num = np.max(np.abs(a - b), axis=-1)
+    den = np.max(np.abs(b), axis=-1)
+    return np.max(num / den)
+
+
+def test_c_integrand_plain_8_fp64():
+    """compute_c_integrand (generalized_bulk_flow.rs:875-920): the time-resolved
+    integrand (2, n_sets, n_w, n_t), no time cutoff, against the oracle; its
+    sum over t equals compute_c_integral."""
+    wl = configs.plain_8()
+    n_c, n_phi, n_t = 16, 32, 60
+    w = wl.w_arr[::4]
+    coef, pows = [[1.0], [0.5]], [[5.0], [4.0]]
+    bf = flow(wl.interior, wl.exterior, n_c, n_phi, coef, pows)
+    I = bf.compute_c_integrand(w, t_end=wl.t_end, n_t=n_t)
+    Io = oracle.compute_c_integrand(wl.interior, wl.exterior, n_c, n_phi, coef, pows, w,
+                                    t_end=wl.t_end, n_t=n_t)
+    assert I.shape == (2, 2, len(w), n_t)
+    assert integrand_rel(I, Io) <= FP64_REL
+    C = bf.compute_c_integral(w, t_end=wl.t_end, n_t=n_t)
+    assert rel_per_bin(I.sum(axis=-1), C) <= 1e-12
+
+
+def test_c_integrand_async_selected():
+    inter, ext = small_async(9, n=8)
+    w = np.geomspace(0.5, 60.0, 10)
+    bf = flow(inter, ext, 12, 20, [[1.0]], [[5.0]])
+    sel = [4, 0, 6]
+    I = bf.compute_c_integrand(w, t_begin=0.05, t_end=0.9, n_t=45, selected_bubbles=sel)
+    Io = oracle.compute_c_integrand(inter, ext, 12, 20, [[1.0]], [[5.0]], w, t_begin=0.05,
+                                    t_end=0.9, n_t=45, selected_bubbles=sel)
+    assert integrand_rel(I, Io) <= FP64_REL
+    # a bubble nucleated after t_end contributes zeros
+    late = int(np.argmax(inter[:, 0]))
+    Z = bf.compute_c_integrand(w, t_end=inter[late, 0] * 0.5, n_t=10, selected_bubbles=[late])
+    assert np.all(Z == 0)
