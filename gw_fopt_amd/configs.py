@@ -12,8 +12,8 @@ images, angular grid, time grid, omegas and k-hat list.
                 64 Fibonacci k-hat; xi = 2.
   mixed_8       plain_8 in the mixed-precision form.
   ensemble_512  512 realisations, beta L ~ U[10, 30], box sides giving ~20-400
-                bubbles (log-uniform targets, mean ~128); 32 x 64; 500 time
-                points to each realisation's percolation; 128 omegas
+                bubbles (log-uniform target counts, mean ~128); 32 x 64; 500
+                time points to each realisation's percolation; 128 omegas
                 geomspace(0.5, 300) / L; 32 k-hat.
   grad_128      128 bubbles, beta L = 15; 32 x 64; 500 time points; 128
                 omegas geomspace(0.5, 300); 32 k-hat; theta = (xi, A) with
@@ -102,18 +102,42 @@ class Ensemble:
         return self.x_arr[None, :] / self.boxes[:, None]
 
 
-def ensemble_512(n_real: int = 512, seed: int = 4, n_k: int = 32) -> Ensemble:
+def _ensemble_member(args):
+    beta, n_target, box, seed = args
+    b = nucleation.exponential_nucleation(beta, box=box, n_max=n_target, seed=seed)
+    return b, nucleation.face_images(b, box), nucleation.percolation_time(b, box=box, n_probe=8192)
+
+
+def ensemble_512(n_real: int = 512, seed: int = 4, n_k: int = 32, workers: int | None = None,
+                 n_min: float = 20.0, n_max: float = 400.0) -> Ensemble:
+    """Realisation r: beta L_r ~ U[10, 30]; a target count n_r log-uniform in
+    [n_min, n_max] (mean ~128 for 20..400); box side for ~1.5 n_r bubbles at
+    percolation (density ~0.68 beta^3 / (8 pi)), nucleation stopped at n_r
+    bubbles; t_end = the realisation's percolation time.  Realisations are
+    generated in parallel worker processes (fork); call this before any GPU
+    work in the process."""
+    import multiprocessing
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+
     rng = np.random.default_rng(seed)
-    interiors, exteriors, boxes, t_end = [], [], [], []
+    jobs = []
     for r in range(n_real):
         beta = rng.uniform(10.0, 30.0)
-        n_target = float(np.exp(rng.uniform(np.log(20.0), np.log(400.0))))
-        # bubbles at percolation ~ 0.7 V beta^3 / (8 pi): box side for the target count
-        box = float((n_target * 8.0 * np.pi / (1.47 * beta ** 3)) ** (1.0 / 3.0))
-        b = nucleation.exponential_nucleation(beta, box=box, n_max=400, seed=seed * 100003 + r)
-        interiors.append(b)
-        exteriors.append(nucleation.face_images(b, box))
-        boxes.append(box)
-        t_end.append(nucleation.percolation_time(b, box=box, n_probe=8192))
-    return Ensemble("ensemble_512", interiors, exteriors, np.array(boxes), np.array(t_end), 32, 64,
-                    500, np.geomspace(0.5, 300.0, 128), fibonacci_directions(n_k))
+        n_target = int(round(float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))))
+        box = float((n_target * 8.0 * np.pi / (0.68 * beta ** 3)) ** (1.0 / 3.0))
+        jobs.append((beta, n_target, box, seed * 100003 + r))
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and n_real > 1:
+        ctx = multiprocessing.get_context("fork")
+        with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+            res = list(ex.map(_ensemble_member, jobs, chunksize=4))
+    else:
+        res = [_ensemble_member(j) for j in jobs]
+    interiors = [r[0] for r in res]
+    exteriors = [r[1] for r in res]
+    t_end = np.array([r[2] for r in res])
+    boxes = np.array([j[2] for j in jobs])
+    return Ensemble("ensemble_512", interiors, exteriors, boxes, t_end, 32, 64, 500,
+                    np.geomspace(0.5, 300.0, 128), fibonacci_directions(n_k))

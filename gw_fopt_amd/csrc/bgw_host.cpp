@@ -20,6 +20,7 @@
 #include <memory>
 #include <numeric>
 #include <stdexcept>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -403,7 +404,14 @@ struct Engine {
   DevBuf d_items, d_order, d_rot, d_sin_t, d_cos_t, d_cos_p, d_sin_p, d_gcos, d_gsin, d_ring_c,
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
       d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
+  // batched ensemble: work units, queue counter, statistics, captured graph
+  DevBuf d_units, d_counter, d_kmean, d_emean, d_estd;
+  hipGraphExec_t graph = nullptr;
+  std::string graph_key;
   BgwTimings timings{};
+  ~Engine() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+  }
 };
 
 void build_static(Engine& E, const std::vector<Bubbles>& reals, size_t n_threads) {
@@ -459,6 +467,9 @@ struct Request {
   int tile_rb = 16;      // time steps per factorised phase block
   bool want_c = true, want_s = true;
   bool skip_uncollided = true;
+  // per-item rows of tau / perm and per-item partial C (false: the fused
+  // ensemble kernel, which keeps neither)
+  bool item_buffers = true;
 };
 
 struct Result {
@@ -487,6 +498,8 @@ struct Launch {
   bgw::SpecArgs sa{};
   bgw::RedArgs ra{};
   size_t n_items = 0, n_groups = 0;
+  std::vector<double> item_cost;         // relative phase-sum cost of each item
+  std::vector<int32_t> group_off, group_r;
 };
 
 void setup(Engine& E, const Request& q, Launch& Lc) {
@@ -550,6 +563,9 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   const size_t nq = n_sets * q.n_w;
   Lc.n_items = n_items;
   Lc.n_groups = n_real * n_k;
+  Lc.item_cost = cost;
+  Lc.group_off = group_off;
+  Lc.group_r = group_r;
   upload(E.d_real, real, st);
   upload(E.d_items, items, st);
   upload(E.d_order, order, st);
@@ -566,9 +582,11 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   upload(E.d_pows, sc.pows, st);
   upload(E.d_group_off, group_off, st);
   upload(E.d_group_r, group_r, st);
-  E.d_tau.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(double));
-  E.d_perm.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(uint16_t));
-  E.d_partial.ensure(std::max<size_t>(1, n_items * 2 * nq) * 2 * sizeof(double));
+  if (q.item_buffers) {
+    E.d_tau.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(double));
+    E.d_perm.ensure(std::max<size_t>(1, n_items * n_dir) * sizeof(uint16_t));
+    E.d_partial.ensure(std::max<size_t>(1, n_items * 2 * nq) * 2 * sizeof(double));
+  }
   E.d_c.ensure(Lc.n_groups * 2 * nq * 2 * sizeof(double));
   E.d_s.ensure(Lc.n_groups * nq * sizeof(double));
 
@@ -796,6 +814,176 @@ std::vector<double> run_integrand(Engine& E, const Request& q) {
   return out;
 }
 
+// ---------------------------------------------------------------------------
+// Batched ensemble: the fused persistent kernel, captured in a hipGraph
+// ---------------------------------------------------------------------------
+struct EnsResult {
+  std::vector<double> s, kmean, mean, stdev;
+  double ms = 0.0;  // device time of one graph replay
+  BgwTileConfig used{0, 0, 0};
+};
+
+int device_cus(Engine& E) {
+  int n = 0;
+  HIPCHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, E.dev.id));
+  return n;
+}
+
+// units: groups in launch order (all groups, largest first, when null).
+// replays >= 1 graph launches; the timing is the mean over them.
+EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
+                       const std::vector<int32_t>* units, bool want_outputs, int replays) {
+  const Resolution& res = *q.res;
+  const Scaling& sc = *q.sc;
+  const size_t n_sets = sc.n_sets;
+  const size_t n_real = E.real.size(), n_k = q.k.size();
+  EnsResult out;
+  const size_t nq = n_sets * q.n_w;
+  Request qq = q;
+  qq.item_buffers = false;
+  qq.tile_threads = static_cast<int>(tiles.threads);
+  qq.tile_rb = tiles.block_steps ? static_cast<int>(tiles.block_steps) : 16;
+  Launch Lc;
+  setup(E, qq, Lc);
+  hipStream_t st = E.dev.stream;
+  const size_t n_dir = res.n_c * res.n_phi;
+
+  // work units: (realisation, k-hat) groups, most expensive first
+  std::vector<int32_t> order;
+  if (units) {
+    order = *units;
+  } else {
+    std::vector<double> gcost(Lc.n_groups, 0.0);
+    for (size_t g = 0; g < Lc.n_groups; ++g) {
+      const size_t r = static_cast<size_t>(Lc.group_r[g]);
+      for (int32_t it = Lc.group_off[g]; it < Lc.group_off[g + 1]; ++it)
+        gcost[g] += Lc.item_cost[it] + 1e-4 * static_cast<double>(E.real[r].n_tot);
+    }
+    order.resize(Lc.n_groups);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return gcost[x] > gcost[y]; });
+  }
+  upload(E.d_units, order, st);
+  E.d_counter.ensure(256);
+  E.d_c.ensure(std::max<size_t>(1, Lc.n_groups * 2 * nq) * 2 * sizeof(double));
+  E.d_s.ensure(std::max<size_t>(1, Lc.n_groups * nq) * sizeof(double));
+  E.d_kmean.ensure(std::max<size_t>(1, n_real * nq) * sizeof(double));
+  E.d_emean.ensure(std::max<size_t>(1, nq) * sizeof(double));
+  E.d_estd.ensure(std::max<size_t>(1, nq) * sizeof(double));
+
+  bgw::EnsArgs ea{};
+  ea.geo = Lc.ga;
+  ea.geo.first_out = nullptr;
+  ea.geo.dtab_out = nullptr;
+  ea.spec = Lc.sa;
+  ea.unit_group = E.d_units.as<int32_t>();
+  ea.group_item_off = E.d_group_off.as<int32_t>();
+  ea.group_r = E.d_group_r.as<int32_t>();
+  ea.n_units = static_cast<int>(order.size());
+  ea.counter = E.d_counter.as<int>();
+  ea.c_out = E.d_c.as<double>();
+  ea.s_out = E.d_s.as<double>();
+
+  // launch shape: one launch per chunk of <= kEnsembleMaxSets sets
+  std::vector<bgw::EnsArgs> launches;
+  int n_blocks = static_cast<int>(tiles.persistent_blocks);
+  for (size_t s0 = 0; s0 < n_sets; s0 += bgw::kEnsembleMaxSets) {
+    bgw::EnsArgs eb = ea;
+    eb.spec.set0 = static_cast<int>(s0);
+    eb.spec.n_sets = static_cast<int>(std::min<size_t>(bgw::kEnsembleMaxSets, n_sets - s0));
+    eb.spec.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
+    eb.spec.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
+    eb.spec.time_tables = 0;
+    eb.spec.time_tables = bgw::spectrum_time_tables(eb.spec) ? 1 : 0;
+    if (bgw::ensemble_lds_bytes(eb, q.mode) + 24 * 1024 > 160 * 1024)
+      throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+    const int per_cu = bgw::ensemble_blocks_per_cu(eb, q.mode);  // also sets the LDS attribute
+    if (per_cu <= 0) throw CalcError("the ensemble kernel cannot be launched with these tiles");
+    if (tiles.persistent_blocks == 0) n_blocks = std::max(n_blocks, per_cu * device_cus(E));
+    launches.push_back(eb);
+  }
+  n_blocks = std::max(n_blocks, 1);
+  E.d_tau.ensure(static_cast<size_t>(n_blocks) * n_dir * sizeof(double));
+  E.d_perm.ensure(static_cast<size_t>(n_blocks) * n_dir * sizeof(uint16_t));
+  for (auto& eb : launches) {
+    eb.geo.tau = E.d_tau.as<double>();
+    eb.geo.perm = E.d_perm.as<uint16_t>();
+    eb.spec.tau = eb.geo.tau;
+    eb.spec.perm = eb.geo.perm;
+  }
+  out.used = tiles;
+  out.used.persistent_blocks = static_cast<size_t>(n_blocks);
+  out.used.block_steps = static_cast<size_t>(qq.tile_rb);
+
+  // graph: [queue reset, persistent kernel] per set chunk, then the statistics
+  std::ostringstream key;
+  key << q.mode << ':' << n_blocks << ':' << order.size() << ':' << n_real << ':' << n_k << ':'
+      << n_sets << ':' << q.n_w << ':' << q.n_t << ':' << qq.tile_threads << ':' << qq.tile_rb;
+  for (const auto& eb : launches)
+    key << ':' << eb.spec.time_tables << ':' << static_cast<const void*>(eb.geo.tau) << ':'
+        << static_cast<const void*>(eb.spec.real) << ':' << static_cast<const void*>(eb.spec.items)
+        << ':' << static_cast<const void*>(eb.spec.omega) << ':'
+        << static_cast<const void*>(eb.spec.coef) << ':'
+        << static_cast<const void*>(eb.unit_group) << ':'
+        << static_cast<const void*>(eb.group_item_off) << ':'
+        << static_cast<const void*>(eb.c_out) << ':' << static_cast<const void*>(eb.s_out) << ':'
+        << static_cast<const void*>(eb.geo.cand) << ':' << static_cast<const void*>(eb.geo.rot);
+  key << ':' << static_cast<const void*>(E.d_kmean.p) << ':' << static_cast<const void*>(E.d_emean.p);
+  if (!E.graph || E.graph_key != key.str()) {
+    if (E.graph) {
+      HIPCHECK(hipGraphExecDestroy(E.graph));
+      E.graph = nullptr;
+    }
+    E.graph_key.clear();
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (const auto& eb : launches) {
+      if (err == hipSuccess) err = bgw::launch_ensemble(eb, q.mode, n_blocks, st);
+    }
+    if (err == hipSuccess)
+      err = bgw::launch_ensemble_stats(E.d_s.as<double>(), static_cast<int>(n_real),
+                                       static_cast<int>(n_k), static_cast<int>(nq),
+                                       E.d_kmean.as<double>(), E.d_emean.as<double>(),
+                                       E.d_estd.as<double>(), st);
+    hipError_t end = hipStreamEndCapture(st, &graph);
+    HIPCHECK(err);
+    HIPCHECK(end);
+    hipError_t inst = hipGraphInstantiate(&E.graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(inst);
+    E.graph_key = key.str();
+  }
+  HIPCHECK(hipEventRecord(E.dev.ev[0], st));
+  for (int rep = 0; rep < std::max(1, replays); ++rep) HIPCHECK(hipGraphLaunch(E.graph, st));
+  HIPCHECK(hipEventRecord(E.dev.ev[1], st));
+  if (want_outputs) {
+    out.s.resize(Lc.n_groups * nq);
+    out.kmean.resize(n_real * nq);
+    out.mean.resize(nq);
+    out.stdev.resize(nq);
+    HIPCHECK(hipMemcpyAsync(out.s.data(), E.d_s.p, out.s.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(out.kmean.data(), E.d_kmean.p, out.kmean.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(out.mean.data(), E.d_emean.p, nq * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(out.stdev.data(), E.d_estd.p, nq * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  }
+  HIPCHECK(hipStreamSynchronize(st));
+  out.ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]) / std::max(1, replays);
+  E.timings.geometry_ms = 0.0;
+  E.timings.spectrum_ms = out.ms;
+  E.timings.reduce_ms = 0.0;
+  E.timings.total_ms = out.ms;
+  E.timings.n_items = Lc.n_items;
+  E.timings.n_directions = Lc.n_items * n_dir;
+  E.timings.n_terms = Lc.n_items * res.n_c * q.n_t * q.n_w;
+  return out;
+}
+
 // Geometry only, for one item: first colliding bubble and delta_tab.
 void run_masks(Engine& E, const Resolution& res, int32_t a, const double* k_hat,
                std::vector<int32_t>& first, std::vector<double>& dtab) {
@@ -862,7 +1050,7 @@ struct bgw_generalized_bulk_flow {
   Resolution res;
   Scaling sc;
   size_t num_threads = 0;
-  BgwTileConfig tiles{0, 0, 0, 0};
+  BgwTileConfig tiles{0, 0, 0};
   Engine engine;
 };
 
@@ -871,7 +1059,7 @@ struct bgw_ensemble {
   Resolution res;
   Scaling sc;
   Engine engine;
-  BgwTileConfig tiles{0, 0, 0, 0};
+  BgwTileConfig tiles{0, 0, 0};
   bool tuned = false;
 };
 
@@ -882,6 +1070,7 @@ void check_tiles(const BgwTileConfig& t) {
     throw CalcError("tiles.block_steps must be 0, 8 or 16");
   if (t.threads != 0 && t.threads != 64 && t.threads != 128 && t.threads != 256)
     throw CalcError("tiles.threads must be 0, 64, 128 or 256");
+  if (t.persistent_blocks > 65536) throw CalcError("tiles.persistent_blocks must be <= 65536");
 }
 
 void apply_tiles(const BgwTileConfig& t, Request& q) {
@@ -1363,7 +1552,7 @@ BgwStatus bgw_generalized_bulk_flow_set_tiles(bgw_generalized_bulk_flow* bulk_fl
                                               const BgwTileConfig* tiles) {
   return run_ffi([&] {
     auto* h = bulk_flow_handle(bulk_flow);
-    BgwTileConfig t{0, 0, 0, 0};
+    BgwTileConfig t{0, 0, 0};
     if (tiles) t = *tiles;
     check_tiles(t);
     h->tiles = t;
@@ -1426,6 +1615,41 @@ BgwStatus bgw_ensemble_set_gradient_scaling_params(bgw_ensemble* ensemble,
   });
 }
 
+namespace {
+
+// Request over every realisation of an ensemble (all bubbles, shared k-hat list).
+Request ensemble_request(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
+                         const double* k_hat, size_t n_k, const double* t_begin,
+                         const double* t_end, size_t n_t, BgwPrecision precision) {
+  const size_t n_real = ensemble->reals.size();
+  slice_from_raw(w_arr, checked_product(n_real, n_w, "w_arr"), "w_arr");
+  slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
+  slice_from_raw(t_begin, n_real, "t_begin");
+  slice_from_raw(t_end, n_real, "t_end");
+  if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+  check_unit_vectors(k_hat, n_k);
+  Request q;
+  q.res = &need_resolution(ensemble->res);
+  q.sc = &ensemble->sc;
+  q.n_w = n_w;
+  q.n_t = n_t;
+  q.identity = false;
+  for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
+  q.omega.assign(w_arr, w_arr + n_real * n_w);
+  q.t_begin.assign(t_begin, t_begin + n_real);
+  q.t_end.assign(t_end, t_end + n_real);
+  for (size_t r = 0; r < n_real; ++r) {
+    check_time_range(q.t_begin[r], q.t_end[r]);
+    std::vector<int32_t> ids(ensemble->reals[r].n_int);
+    std::iota(ids.begin(), ids.end(), 0);
+    q.bubbles_of_real.push_back(ids);
+  }
+  q.mode = kernel_mode(precision);
+  return q;
+}
+
+}  // namespace
+
 BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
                                         const double* k_hat, size_t n_k, const double* t_begin,
                                         const double* t_end, size_t n_t, BgwPrecision precision,
@@ -1433,7 +1657,6 @@ BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_
                                         size_t spectrum_len, double* kmean_out,
                                         size_t kmean_len, double* ens_mean_out,
                                         double* ens_std_out, size_t ens_len) {
-  (void)tiles;
   if (!valid_precision(precision)) {
     set_last_error("precision is not a BgwPrecision");
     return BgwStatus_InvalidArgument;
@@ -1441,61 +1664,29 @@ BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_
   return run_ffi([&] {
     if (!ensemble) throw CalcError(null_error("ensemble"));
     const size_t n_real = ensemble->reals.size();
-    slice_from_raw(w_arr, checked_product(n_real, n_w, "w_arr"), "w_arr");
-    slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
-    slice_from_raw(t_begin, n_real, "t_begin");
-    slice_from_raw(t_end, n_real, "t_end");
-    const size_t n_sets = ensemble->sc.n_sets;
-    const size_t nq = n_sets * n_w;
+    const size_t nq = ensemble->sc.n_sets * n_w;
     if (spectrum_len != n_real * n_k * nq)
       throw CalcError("spectrum_len=" + std::to_string(spectrum_len) +
                       " does not match expected length " + std::to_string(n_real * n_k * nq));
     slice_from_raw(spectrum_out, spectrum_len, "spectrum_out");
     if (kmean_out && kmean_len != n_real * nq) throw CalcError("kmean_len does not match");
     if ((ens_mean_out || ens_std_out) && ens_len != nq) throw CalcError("ens_len does not match");
-    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
-    const Resolution& res = need_resolution(ensemble->res);
-    Request q;
-    q.res = &res;
-    q.sc = &ensemble->sc;
-    q.n_w = n_w;
-    q.n_t = n_t;
-    q.identity = false;
-    for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
-    q.omega.assign(w_arr, w_arr + n_real * n_w);
-    q.t_begin.assign(t_begin, t_begin + n_real);
-    q.t_end.assign(t_end, t_end + n_real);
-    for (size_t r = 0; r < n_real; ++r) {
-      check_time_range(q.t_begin[r], q.t_end[r]);
-      std::vector<int32_t> ids(ensemble->reals[r].n_int);
-      std::iota(ids.begin(), ids.end(), 0);
-      q.bubbles_of_real.push_back(ids);
+    BgwTileConfig t = tiles ? *tiles : ensemble->tiles;
+    check_tiles(t);
+    Request q = ensemble_request(ensemble, w_arr, n_w, k_hat, n_k, t_begin, t_end, n_t, precision);
+    EnsResult er;
+    if (n_k > 0 && n_real > 0 && nq > 0) {
+      er = run_ensemble(ensemble->engine, q, t, nullptr, true, 1);
+    } else {
+      er.s.assign(n_real * n_k * nq, 0.0);
+      er.kmean.assign(n_real * nq, 0.0);
+      er.mean.assign(nq, 0.0);
+      er.stdev.assign(nq, 0.0);
     }
-    q.mode = kernel_mode(precision);
-    std::vector<double> S(n_real * n_k * nq, 0.0);
-    if (n_k > 0 && n_real > 0) {
-      Result r = run(ensemble->engine, q);
-      S = std::move(r.s);
-    }
-    std::memcpy(spectrum_out, S.data(), S.size() * sizeof(double));
-    std::vector<double> km(n_real * nq, 0.0);
-    for (size_t r = 0; r < n_real; ++r)
-      for (size_t j = 0; j < nq; ++j) {
-        double acc = 0.0;
-        for (size_t k = 0; k < n_k; ++k) acc += S[(r * n_k + k) * nq + j];
-        km[r * nq + j] = n_k ? acc / static_cast<double>(n_k) : 0.0;
-      }
-    if (kmean_out) std::memcpy(kmean_out, km.data(), km.size() * sizeof(double));
-    for (size_t j = 0; j < nq && (ens_mean_out || ens_std_out); ++j) {
-      double m = 0.0;
-      for (size_t r = 0; r < n_real; ++r) m += km[r * nq + j];
-      m = n_real ? m / static_cast<double>(n_real) : 0.0;
-      double v = 0.0;
-      for (size_t r = 0; r < n_real; ++r) v += (km[r * nq + j] - m) * (km[r * nq + j] - m);
-      v = n_real ? v / static_cast<double>(n_real) : 0.0;
-      if (ens_mean_out) ens_mean_out[j] = m;
-      if (ens_std_out) ens_std_out[j] = std::sqrt(v);
-    }
+    std::memcpy(spectrum_out, er.s.data(), er.s.size() * sizeof(double));
+    if (kmean_out) std::memcpy(kmean_out, er.kmean.data(), er.kmean.size() * sizeof(double));
+    if (ens_mean_out) std::memcpy(ens_mean_out, er.mean.data(), nq * sizeof(double));
+    if (ens_std_out) std::memcpy(ens_std_out, er.stdev.data(), nq * sizeof(double));
   });
 }
 
@@ -1503,11 +1694,64 @@ BgwStatus bgw_ensemble_autotune(bgw_ensemble* ensemble, const double* w_arr, siz
                                 const double* k_hat, size_t n_k, const double* t_begin,
                                 const double* t_end, size_t n_t, BgwPrecision precision,
                                 BgwTileConfig* best) {
-  (void)w_arr; (void)n_w; (void)k_hat; (void)n_k; (void)t_begin; (void)t_end; (void)n_t;
-  (void)precision;
+  if (!valid_precision(precision)) {
+    set_last_error("precision is not a BgwPrecision");
+    return BgwStatus_InvalidArgument;
+  }
   return run_ffi([&] {
     if (!ensemble) throw CalcError(null_error("ensemble"));
-    if (best) *best = ensemble->tiles;
+    Request q = ensemble_request(ensemble, w_arr, n_w, k_hat, n_k, t_begin, t_end, n_t, precision);
+    const size_t n_groups = ensemble->reals.size() * n_k;
+    if (n_groups == 0 || n_w == 0 || ensemble->sc.n_sets == 0) {
+      if (best) *best = ensemble->tiles;
+      return;
+    }
+    // Candidates: workgroup size (omegas per lane 1, 2 or 4) x block length;
+    // each is timed on an evenly spaced sample of the largest-first units.
+    std::vector<size_t> threads;
+    for (size_t opl : {1, 2, 4}) {
+      const size_t t = ((n_w + opl - 1) / opl + 63) / 64 * 64;
+      if (t >= 64 && t <= 256 && (n_w + t - 1) / t <= 4 &&
+          std::find(threads.begin(), threads.end(), t) == threads.end())
+        threads.push_back(t);
+    }
+    const std::vector<size_t> blocks = q.mode == 0 ? std::vector<size_t>{16}
+                                                   : std::vector<size_t>{8, 16};
+    // sample: every m-th unit of the cost order, at most ~4096 units
+    std::vector<int32_t> sample;
+    {
+      Launch Lc;
+      Request qq = q;
+      qq.item_buffers = false;
+      setup(ensemble->engine, qq, Lc);
+      std::vector<double> gcost(Lc.n_groups, 0.0);
+      for (size_t g = 0; g < Lc.n_groups; ++g)
+        for (int32_t it = Lc.group_off[g]; it < Lc.group_off[g + 1]; ++it)
+          gcost[g] += Lc.item_cost[it] +
+                      1e-4 * static_cast<double>(ensemble->engine.real[Lc.group_r[g]].n_tot);
+      std::vector<int32_t> order(Lc.n_groups);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int32_t x, int32_t y) { return gcost[x] > gcost[y]; });
+      const size_t m = std::max<size_t>(1, order.size() / 4096);
+      for (size_t i = 0; i < order.size(); i += m) sample.push_back(order[i]);
+    }
+    double best_ms = std::numeric_limits<double>::infinity();
+    BgwTileConfig win = ensemble->tiles;
+    for (size_t t : threads)
+      for (size_t rb : blocks) {
+        BgwTileConfig cand{rb, t, 0};
+        (void)run_ensemble(ensemble->engine, q, cand, &sample, false, 1);  // capture + warm-up
+        EnsResult r = run_ensemble(ensemble->engine, q, cand, &sample, false, 2);
+        if (r.ms < best_ms) {
+          best_ms = r.ms;
+          win = r.used;
+          win.persistent_blocks = 0;
+        }
+      }
+    ensemble->tiles = win;
+    ensemble->tuned = true;
+    if (best) *best = win;
   });
 }
 

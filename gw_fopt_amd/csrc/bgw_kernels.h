@@ -69,6 +69,31 @@ struct RedArgs {
   double* s_out;             // (n_groups, n_sets, n_w)
 };
 
+// Batched ensemble: one persistent launch over (realisation, k-hat) groups.
+struct EnsArgs {
+  GeoArgs geo;                    // tau / perm: per-workgroup scratch rows (n_blocks, n_c * n_phi)
+  SpecArgs spec;                  // tau / perm as geo; partial unused; n_sets <= 2
+  const int32_t* unit_group;      // (n_units) groups in launch order
+  const int32_t* group_item_off;  // (n_groups + 1) items of each group, in bubble order
+  const int32_t* group_r;         // (n_groups) realisation of each group
+  int n_units;
+  int* counter;                   // work-queue head, cleared by launch_ensemble
+  double* c_out;                  // (n_groups, 2, n_sets_out, n_w) complex
+  double* s_out;                  // (n_groups, n_sets_out, n_w)
+};
+constexpr int kEnsembleMaxSets = 2;
+// Resident workgroups per CU for this launch shape (0: cannot launch); also
+// sets the kernel's dynamic-LDS limit, so call it before launch_ensemble.
+int ensemble_blocks_per_cu(const EnsArgs& a, int mode);
+// Clears the queue counter (a kernel, so the pair can be graph-captured) and
+// launches n_blocks persistent workgroups.
+hipError_t launch_ensemble(const EnsArgs& a, int mode, int n_blocks, hipStream_t s);
+size_t ensemble_lds_bytes(const EnsArgs& a, int mode);
+// kmean (n_real, nq) = mean over k of s (n_real, n_k, nq); if mean: mean and
+// population standard deviation over realisations of kmean.
+hipError_t launch_ensemble_stats(const double* s_spec, int n_real, int n_k, int nq, double* kmean,
+                                 double* mean, double* stdev, hipStream_t s);
+
 hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s);
 hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s);
 hipError_t launch_reduce(const RedArgs& a, int n_groups, hipStream_t s);

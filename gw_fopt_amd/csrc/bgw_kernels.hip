@@ -31,106 +31,9 @@
 //   (deterministic) and form S = 8 pi w^3 (|C+|^2 + |Cx|^2).
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
-#include "bgw_common.h"
-#include "bgw_device.h"
-#include "bgw_kernels.h"
+#include "bgw_stages.h"
 
 namespace bgw {
-
-namespace {
-
-constexpr int kGeoThreads = 256;
-constexpr int kGeoLdsCand = 128;
-constexpr int kSortCap = 1024;
-constexpr double kInf = __builtin_inf();
-
-// A candidate in the k-hat frame, as staged in LDS.
-struct alignas(16) RotCand {
-  double d0, d1, d2, d3;  // rotated delta
-  double dsq;             // stored delta_squared (unrotated)
-  double m;               // scalar(rotated delta, rotated delta)
-  double key;
-  int32_t idx, flags;
-};
-static_assert(sizeof(RotCand) == 64, "");
-
-__device__ __forceinline__ RotCand rotate_entry(const CandEntry& e, const double* R, int identity) {
-  RotCand o;
-  o.d0 = e.d[0];
-  if (identity) {
-    o.d1 = e.d[1];
-    o.d2 = e.d[2];
-    o.d3 = e.d[3];
-  } else {
-    o.d1 = (R[0] * e.d[1] + R[1] * e.d[2]) + R[2] * e.d[3];
-    o.d2 = (R[3] * e.d[1] + R[4] * e.d[2]) + R[5] * e.d[3];
-    o.d3 = (R[6] * e.d[1] + R[7] * e.d[2]) + R[8] * e.d[3];
-  }
-  o.dsq = e.dsq;
-  o.m = lscalar(o.d0, o.d1, o.d2, o.d3, o.d0, o.d1, o.d2, o.d3);
-  o.key = e.key;
-  o.idx = e.idx;
-  o.flags = e.flags;
-  return o;
-}
-
-// check_collision_point (:1003-1018): true iff b reaches the point on a's
-// wall along x before c does.  b.m / c.m are the recomputed norms.
-__device__ __forceinline__ bool collision_point_ok(const RotCand& b, const RotCand& c, double x1,
-                                                   double x2, double x3) {
-  double dbx = lscalar(b.d0, b.d1, b.d2, b.d3, 1.0, x1, x2, x3);
-  if (dbx <= 0.0) return false;
-  double nba = b.m, nca = c.m;
-  double v0 = b.d0 * nca - c.d0 * nba;
-  double v1 = b.d1 * nca - c.d1 * nba;
-  double v2 = b.d2 * nca - c.d2 * nba;
-  double v3 = b.d3 * nca - c.d3 * nba;
-  return lscalar(v0, v1, v2, v3, 1.0, x1, x2, x3) > 0.0;
-}
-
-__device__ __forceinline__ RotCand fetch_cand(int s, const RotCand* lds, int n_lds,
-                                              const CandEntry* list, const double* R,
-                                              int identity) {
-  if (s < n_lds) return lds[s];
-  return rotate_entry(list[s], R, identity);
-}
-
-// The reference's scan in original bubble order (:162-199).  Used for lanes
-// whose fast path cannot certify its answer.
-__device__ int reference_scan(const CandEntry* list, const int32_t* inv, int a, int n_int,
-                              int n_tot, const double* R, int identity, double x1, double x2,
-                              double x3) {
-  double best = kInf;
-  int bidx = -1;
-  for (int b = 0; b < n_tot; ++b) {
-    if (b < n_int && b == a) continue;
-    RotCand eb = rotate_entry(list[inv[b]], R, identity);
-    double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1, x2, x3);
-    if (dot <= 1e-10) continue;
-    double dtv = eb.dsq / (2.0 * dot);
-    if (dtv <= 0.0 || dtv >= best) continue;
-    bool first = true;
-    for (int c = 0; c < n_tot; ++c) {
-      if ((c < n_int && c == a) || c == b) continue;
-      RotCand ec = rotate_entry(list[inv[c]], R, identity);
-      if (!collision_point_ok(eb, ec, x1, x2, x3)) {
-        first = false;
-        break;
-      }
-    }
-    if (first) {
-      bidx = b;
-      best = dtv;
-    }
-  }
-  return bidx;
-}
-
-__device__ __forceinline__ bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Stage 1: first collisions
@@ -139,598 +42,21 @@ __global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
   __shared__ RotCand cand_lds[kGeoLdsCand];
   __shared__ double sort_key[kSortCap];
   __shared__ int32_t sort_val[kSortCap];
-
   const int item_id = blockIdx.x;
-  const Item it = g.items[item_id];
-  const RealInfo ri = g.real[it.r];
-  const int n_cand = ri.n_tot - 1;
-  const CandEntry* list = g.cand + ri.cand_off + static_cast<int64_t>(it.a) * n_cand;
-  const int32_t* inv = g.inv + ri.inv_off + static_cast<int64_t>(it.a) * ri.n_tot;
-  double R[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) R[q] = g.identity ? 0.0 : g.rot[9 * it.k + q];
-  const int identity = g.identity;
-
-  const int n_lds = n_cand < kGeoLdsCand ? n_cand : kGeoLdsCand;
-  for (int s = threadIdx.x; s < n_lds; s += blockDim.x) cand_lds[s] = rotate_entry(list[s], R, identity);
-  __syncthreads();
-
-  const int n_phi = g.n_phi;
-  const int P = g.n_phi_pow2;
-  const int G = kSortCap / P;  // rings per sort group
-  const int64_t n_dir = static_cast<int64_t>(g.n_c) * n_phi;
-  double* tau_out = g.tau + static_cast<int64_t>(item_id) * n_dir;
-  uint16_t* perm_out = g.perm + static_cast<int64_t>(item_id) * n_dir;
-
-  // Each wavefront takes a compact patch of ph rings x pw azimuths, so its
-  // lanes share first colliders and stop their culled scans together.
-  const int pw = P < 8 ? P : 8;
-  const int ph = 64 / pw;
-  const int patches_per_row = P / pw;
-  for (int i0 = 0; i0 < g.n_c; i0 += G) {
-    const int gcount = min(G, g.n_c - i0);
-    const int nslot = gcount * P;
-    const int n_patch = ((gcount + ph - 1) / ph) * patches_per_row;
-    const int nwork_round = ((n_patch * 64 + blockDim.x - 1) / blockDim.x) * blockDim.x;
-    for (int w = threadIdx.x; w < nwork_round; w += blockDim.x) {
-      const int patch = w >> 6, lane = w & 63;
-      const int prow = patch / patches_per_row, pcol = patch % patches_per_row;
-      const int rring = prow * ph + lane / pw;  // ring within the group
-      const int j = pcol * pw + lane % pw;
-      const int ring = i0 + rring;
-      const int slot = rring * P + j;           // sort-buffer position
-      const bool in_group = patch < n_patch && rring < gcount;
-      const bool active = in_group && j < n_phi;
-      double x1 = 0.0, x2 = 0.0, x3 = 0.0;
-      if (active) {
-        const double st = g.sin_t[ring];
-        x1 = st * g.cos_p[j];
-        x2 = st * g.sin_p[j];
-        x3 = g.cos_t[ring];
-      }
-      bool fb = active && (ri.nonfinite || !(isfinite(x1) && isfinite(x2) && isfinite(x3)));
-      const bool act = active && !fb;
-
-      // Pass 1: minimum collision delay over the candidates (dot > 1e-10,
-      // delay > 0), ties to the lowest bubble index.  The list is sorted by a
-      // lower bound `key` of the delay: once key * (1 - 1e-7) > best for every
-      // lane, no later candidate can reach or tie a lane's best.
-      double best = kInf;
-      int bidx = 0x7fffffff, bpos = -1;
-      for (int s = 0; s < n_cand; ++s) {
-        const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
-        const bool need = act && !(e.key * (1.0 - 1e-7) > best);
-        if (!wave_any(need)) break;
-        if (need) {
-          const double dot = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1, x2, x3);
-          if (!(dot <= 1e-10)) {
-            const double dtv = e.dsq / (2.0 * dot);
-            if (dtv != dtv) {
-              fb = true;
-            } else if (dtv > 0.0 && (dtv < best || (dtv == best && e.idx < bidx))) {
-              best = dtv;
-              bidx = e.idx;
-              bpos = s;
-            }
-          }
-        }
-      }
-
-      // Pass 2: verify the minimum against every other bubble c with the
-      // reference's cross-multiplied test.  For a regular, non-grazing
-      // candidate, bubbles with key > best * (1 + 1e-4) provably pass, so the
-      // wave again walks a prefix of the list.
-      const bool p2 = act && !fb && bpos >= 0;
-      RotCand b0{};
-      double thr = kInf;
-      if (p2) {
-        b0 = fetch_cand(bpos, cand_lds, n_lds, list, R, identity);
-        const double scale = fabs(b0.d0) + sqrt(b0.d1 * b0.d1 + b0.d2 * b0.d2 + b0.d3 * b0.d3);
-        const bool regular = (b0.flags & kCandRegular) != 0;
-        const bool grazing = !(best <= 1e3 * scale);
-        if (regular && !grazing) thr = best * (1.0 + 1e-4);
-      }
-      bool ok = true;
-      for (int s = 0; s < n_cand; ++s) {
-        const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
-        const bool need = p2 && ok && !(e.key > thr);
-        if (!wave_any(need)) break;
-        if (need && s != bpos && !collision_point_ok(b0, e, x1, x2, x3)) ok = false;
-      }
-      if (p2 && !ok) fb = true;
-
-      int result = -1;
-      if (fb) {
-        result = reference_scan(list, inv, it.a, ri.n_int, ri.n_tot, R, identity, x1, x2, x3);
-      } else if (p2) {
-        result = bidx;
-      }
-      double tau = kInf;
-      if (active) {
-        double dtab = 0.0;  // compute_delta_tab (:446-468)
-        if (result >= 0) {
-          const RotCand eb = fetch_cand(inv[result], cand_lds, n_lds, list, R, identity);
-          const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1, x2, x3);
-          dtab = fabs(dot) < 1e-10 ? 0.0 : eb.dsq / (2.0 * dot);
-          if (dtab > 0.0) tau = dtab;  // AlreadyCollided needs delta_tab > 0 (:405)
-        }
-        const int64_t dir = static_cast<int64_t>(ring) * n_phi + j;
-        if (g.first_out) {
-          int code = -1;
-          if (result >= 0) code = result < ri.n_int ? result : -(result - ri.n_int) - 2;
-          g.first_out[static_cast<int64_t>(item_id) * n_dir + dir] = code;
-        }
-        if (g.dtab_out) g.dtab_out[static_cast<int64_t>(item_id) * n_dir + dir] = dtab;
-      }
-      if (in_group) {
-        sort_key[slot] = tau;
-        sort_val[slot] = j;
-      }
-    }
-    __syncthreads();
-    // Bitonic sort of every P-slot segment (one ring each), ascending in
-    // (delay, phi index).  Padding slots carry phi index >= n_phi, so they
-    // sort after the directions that never collide (delay +inf as well).
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int jj = k >> 1; jj > 0; jj >>= 1) {
-        for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
-          const int p = e ^ jj;
-          if (p > e) {
-            // direction from the index within the ring's segment, so every
-            // segment ends ascending (k == P merges the whole segment)
-            const bool up = ((e & (P - 1)) & k) == 0;
-            const double ke = sort_key[e], kp = sort_key[p];
-            const bool greater = ke > kp || (ke == kp && sort_val[e] > sort_val[p]);
-            if (greater == up) {
-              sort_key[e] = kp;
-              sort_key[p] = ke;
-              const int32_t t = sort_val[e];
-              sort_val[e] = sort_val[p];
-              sort_val[p] = t;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
-      const int ring = i0 + e / P;
-      const int r = e % P;
-      if (r < n_phi) {
-        tau_out[static_cast<int64_t>(ring) * n_phi + r] = sort_key[e];
-        perm_out[static_cast<int64_t>(ring) * n_phi + r] = static_cast<uint16_t>(sort_val[e]);
-      }
-    }
-    __syncthreads();
-  }
+  const int64_t n_dir = static_cast<int64_t>(g.n_c) * g.n_phi;
+  geometry_item(g, item_id, g.tau + item_id * n_dir, g.perm + item_id * n_dir, cand_lds, sort_key,
+                sort_val);
 }
 
 // ---------------------------------------------------------------------------
-// Stage 2 helpers
+// Stage 2: phase sums, one workgroup per item
 // ---------------------------------------------------------------------------
-namespace {
-
-// Inclusive scan of arr[0..n) by one whole wavefront (reverse: suffix sums).
-__device__ void wave_scan(double* arr, int n, bool reverse) {
-  const int lane = threadIdx.x & 63;
-  double carry = 0.0;
-  for (int base = 0; base < n; base += 64) {
-    const int e = base + lane;
-    const int idx = reverse ? (n - 1 - e) : e;
-    double v = e < n ? arr[idx] : 0.0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const double u = __shfl_up(v, off, 64);
-      if (lane >= off) v += u;
-    }
-    v += carry;
-    if (e < n) arr[idx] = v;
-    carry = __shfl(v, 63, 64);
-  }
-}
-
-__device__ __forceinline__ int count_le(const double* sorted, int n, double x) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (sorted[mid] <= x) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ double cutoff_weight(const SpecArgs& g, double t) {
-  if (g.cutoff_kind == 0 || t < g.t_cut) return 1.0;
-  const double d = t - g.t_cut;
-  return exp(-(d * d) * g.inv_t0_sq);
-}
-
-constexpr int kTimeChunk = 256;
-
-// Per-ring tables in LDS, laid out after each other, then the per-item time
-// tables (when they fit) and the per-chunk phase inputs.
-struct RingLds {
-  double* tau;  // [n_phi] sorted delays
-  double* gc;   // [n_phi] w_j cos 2phi_j in sorted order
-  double* gs;   // [n_phi]
-  double* uc;   // [n_phi + 1] suffix sums of gc (uncollided part)
-  double* us;   // [n_phi + 1]
-  double* cc;   // [NT][n_phi + 1] prefix sums of gc * tau^p (collided part)
-  double* cs;   // [NT][n_phi + 1]
-  double* cl;   // [NT][n_phi + 1] prefix sums of gc * tau^p ln tau (adjoint only)
-  double* cls;  // [NT][n_phi + 1]
-  double* tb;   // [n_t] K dt w_t cutoff(t) (t - t_a)^3, or null
-  double* tf;   // [NT][n_t] coef_q (t - t_a)^-p_q, or null
-  double* psi;  // [kTimeChunk] phase argument per time step
-  double* v;    // [kTimeChunk][2 * n_sets] weights V+ / Vx per time step
-  float* vf;    // the same weights in float (mixed form), aliases v
-  int stride;   // n_phi + 1
-};
-
-__host__ __device__ inline size_t ring_lds_doubles(int n_phi, int nt_terms) {
-  return 3 * static_cast<size_t>(n_phi) + 2 * static_cast<size_t>(n_phi + 1) +
-         2 * static_cast<size_t>(nt_terms) * (n_phi + 1);
-}
-
-__host__ __device__ inline size_t time_table_doubles(int n_t, int nt_terms) {
-  return static_cast<size_t>(n_t) * (1 + nt_terms);
-}
-
-__device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool time_tables) {
-  RingLds L;
-  L.stride = n_phi + 1;
-  double* p = smem;
-  L.tau = p; p += n_phi;
-  L.gc = p; p += n_phi;
-  L.gs = p; p += n_phi;
-  L.uc = p; p += L.stride;
-  L.us = p; p += L.stride;
-  L.cc = p; p += static_cast<size_t>(nt_terms) * L.stride;
-  L.cs = p; p += static_cast<size_t>(nt_terms) * L.stride;
-  L.cl = nullptr;
-  L.cls = nullptr;
-  L.tb = nullptr;
-  L.tf = nullptr;
-  if (time_tables) {
-    L.tb = p; p += n_t;
-    L.tf = p; p += static_cast<size_t>(nt_terms) * n_t;
-  }
-  L.psi = p; p += kTimeChunk;
-  L.v = p;
-  L.vf = reinterpret_cast<float*>(p);
-  return L;
-}
-
-// Loads ring i of an item and builds its prefix/suffix tables.  Returns the
-// first time index with a contribution and whether the ring is skipped.
-__device__ int prepare_ring(const SpecArgs& g, const RingLds& L, int64_t item_id, int i,
-                            double t_a, const RealInfo& ri, bool* skip) {
-  const int n_phi = g.n_phi;
-  const int nterms = g.n_sets * g.n_terms;
-  const int64_t base = (item_id * g.n_c + i) * static_cast<int64_t>(n_phi);
-  for (int r = threadIdx.x; r < n_phi; r += blockDim.x) {
-    const double tau = g.tau[base + r];
-    const int j = g.perm[base + r];
-    const double gcv = g.gcos[j], gsv = g.gsin[j];
-    L.tau[r] = tau;
-    L.gc[r] = gcv;
-    L.gs[r] = gsv;
-    L.uc[r] = gcv;
-    L.us[r] = gsv;
-    const bool fin = tau < kInf;
-    for (int q = 0; q < nterms; ++q) {
-      const double pw = fin ? pow(tau, g.pows[q]) : 0.0;
-      L.cc[q * L.stride + r + 1] = gcv * pw;
-      L.cs[q * L.stride + r + 1] = gsv * pw;
-      if (L.cl) {
-        const double lw = fin ? pw * log(tau) : 0.0;
-        L.cl[q * L.stride + r + 1] = gcv * lw;
-        L.cls[q * L.stride + r + 1] = gsv * lw;
-      }
-    }
-  }
-  if (threadIdx.x == 0) {
-    L.uc[n_phi] = 0.0;
-    L.us[n_phi] = 0.0;
-    for (int q = 0; q < nterms; ++q) {
-      L.cc[q * L.stride] = 0.0;
-      L.cs[q * L.stride] = 0.0;
-      if (L.cl) {
-        L.cl[q * L.stride] = 0.0;
-        L.cls[q * L.stride] = 0.0;
-      }
-    }
-  }
-  __syncthreads();
-  const int n_waves = blockDim.x >> 6;
-  const int wave = threadIdx.x >> 6;
-  const int n_prefix = 2 * nterms * (L.cl ? 2 : 1);
-  const int n_arrays = 2 + n_prefix;
-  for (int arr = wave; arr < n_arrays; arr += n_waves) {
-    if (arr == 0) wave_scan(L.uc, n_phi, true);
-    else if (arr == 1) wave_scan(L.us, n_phi, true);
-    else {
-      const int e = arr - 2;
-      const int q = (e >> 1) % nterms;
-      const bool logs = e >= 2 * nterms;
-      double* base_arr = (e & 1) ? (logs ? L.cls : L.cs) : (logs ? L.cl : L.cc);
-      wave_scan(base_arr + q * L.stride + 1, n_phi, false);
-    }
-  }
-  __syncthreads();
-  // A ring that never collides contributes B = 0.5 sin^2 dphi dt^3 sum_j w_j
-  // cos(2 phi_j) (resp. sin), a full trapezoid period that vanishes up to
-  // rounding for n_phi >= 4.  Such rings, and the time steps before a ring's
-  // first collision, are skipped when their sum is at the rounding level.
-  const double tau_min = L.tau[0];
-  const bool noise = fabs(L.uc[0]) <= 1e-12 * n_phi && fabs(L.us[0]) <= 1e-12 * n_phi;
-  const bool drop_early = g.skip_uncollided && noise && !g.has_damping;
-  *skip = drop_early && !(tau_min < kInf);
-  // first k with t_k - t_a > 0 (and >= tau_min when dropping early steps)
-  int lo = 0, hi = g.n_t;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const double t = ri.t_begin + ri.t_step * static_cast<double>(mid);
-    const double dta = t - t_a;
-    const bool pred = dta > 0.0 && (!drop_early || dta >= tau_min);
-    if (pred) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
-}
-
-// K dt w_t cutoff(t) (t - t_a)^3 and coef_q (t - t_a)^-p_q for every time step of
-// an item (ring-independent parts of the weights).  Zero where t <= t_a.
-__device__ void prepare_item_tables(const SpecArgs& g, const RingLds& L, const Item& it,
-                                    const RealInfo& ri) {
-  const int nterms = g.n_sets * g.n_terms;
-  for (int k = threadIdx.x; k < g.n_t; k += blockDim.x) {
-    const double t = ri.t_begin + ri.t_step * static_cast<double>(k);
-    const double dta = t - it.t_a;
-    double base = 0.0;
-    if (dta > 0.0) {
-      const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
-      base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta);
-    }
-    L.tb[k] = base;
-    for (int q = 0; q < nterms; ++q)
-      L.tf[static_cast<size_t>(q) * g.n_t + k] = dta > 0.0 ? g.coef[q] * pow(dta, -g.pows[q]) : 0.0;
-  }
-}
-
-// V_pm(t) and psi(t) for the time steps [t0, t0 + len) of ring i.  FLOAT_V:
-// the weights are stored in float (mixed form).
-template <bool FLOAT_V>
-__device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0, int len,
-                              const Item& it, const RealInfo& ri) {
-  const int n_phi = g.n_phi;
-  const int ns = g.n_sets;
-  const double c_i = g.ring_c[i];
-  const double rf = g.ring_fac[i];
-  for (int l = threadIdx.x; l < kTimeChunk; l += blockDim.x) {
-    double vv[2 * 8];
-    for (int s = 0; s < 2 * ns; ++s) vv[s] = 0.0;
-    double psi = 0.0;
-    if (l < len) {
-      const int k = t0 + l;
-      const double t = ri.t_begin + ri.t_step * static_cast<double>(k);
-      const double dta = t - it.t_a;
-      psi = (t - it.z_a) - dta * c_i;
-      if (dta > 0.0) {  // compute_b_integral returns zeros for dta <= 0 (:541-543)
-        if (!g.has_damping) {
-          const int cnt = count_le(L.tau, n_phi, dta);
-          double base;
-          if (L.tb) {
-            base = L.tb[k] * rf;
-          } else {
-            const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
-            base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta) * rf;
-          }
-          for (int s = 0; s < ns; ++s) {
-            double bp = L.uc[cnt], bx = L.us[cnt];
-            for (int kk = 0; kk < g.n_terms; ++kk) {
-              const int q = s * g.n_terms + kk;
-              const double f = L.tf ? L.tf[static_cast<size_t>(q) * g.n_t + k]
-                                    : g.coef[q] * pow(dta, -g.pows[q]);
-              bp = __builtin_fma(f, L.cc[q * L.stride + cnt], bp);
-              bx = __builtin_fma(f, L.cs[q * L.stride + cnt], bx);
-            }
-            vv[2 * s + 0] = base * bp;
-            vv[2 * s + 1] = base * bx;
-          }
-        } else {
-          // Direct phi sum with the damping factor exp(-dta (1 - ratio) / width).
-          const double wt = (k == 0 || k == g.n_t - 1) ? 0.5 : 1.0;
-          const double base = g.kfac * ri.dt * wt * cutoff_weight(g, t) * ((dta * dta) * dta) * rf;
-          for (int s = 0; s < ns; ++s) {
-            double bp = 0.0, bx = 0.0;
-            for (int r = 0; r < n_phi; ++r) {
-              const double tau = L.tau[r];
-              double f = 1.0;
-              if (tau <= dta) {
-                const double ratio = tau / dta;
-                f = 0.0;
-                for (int kk = 0; kk < g.n_terms; ++kk) {
-                  const int q = s * g.n_terms + kk;
-                  f += g.coef[q] * pow(ratio, g.pows[q]);
-                }
-                f *= exp((-dta) * (1.0 - ratio) / g.damping);
-              }
-              bp = __builtin_fma(L.gc[r], f, bp);
-              bx = __builtin_fma(L.gs[r], f, bx);
-            }
-            vv[2 * s + 0] = base * bp;
-            vv[2 * s + 1] = base * bx;
-          }
-        }
-      }
-    }
-    L.psi[l] = psi;
-    if constexpr (FLOAT_V) {
-      for (int s = 0; s < 2 * ns; ++s) L.vf[l * 2 * ns + s] = static_cast<float>(vv[s]);
-    } else {
-      for (int s = 0; s < 2 * ns; ++s) L.v[l * 2 * ns + s] = vv[s];
-    }
-  }
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// Stage 2: phase sums
-// ---------------------------------------------------------------------------
-// MODE 0: FP64, one double sincos per (t, omega) term (the parity form).
-// MODE 1: FP64, block-factorised phases: exp(i w psi(t0 + r)) = A(t0) E_r with
-//         E_r = exp(i w r dt (1 - cos th)); double E_r (recurrence), double
-//         anchors A, double block sums.
-// MODE 2: mixed, the same factorisation with the phase arguments reduced in
-//         double and the sincos, E_r table, weights and block sums in float;
-//         block results accumulated in float with Kahan compensation.
 template <int NS, int OPL, int MODE, int RB>
 __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
-  using Acc = typename std::conditional<MODE == 2, float, double>::type;
   extern __shared__ double smem[];
   const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
-  const Item it = g.items[item_id];
-  const RealInfo ri = g.real[it.r];
-  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, g.time_tables != 0);
-  if (L.tb) prepare_item_tables(g, L, it, ri);
-  __syncthreads();
-
-  double om[OPL];
-#pragma unroll
-  for (int q = 0; q < OPL; ++q) {
-    const int wi = threadIdx.x + q * blockDim.x;
-    om[q] = wi < g.n_w ? g.omega[ri.w_off + wi] : 0.0;
-  }
-  // running sums (double), or float sums + Kahan compensations in MODE 2
-  Acc acc[OPL][NS][4];
-  float comp[MODE == 2 ? OPL : 1][MODE == 2 ? NS : 1][4];
-#pragma unroll
-  for (int q = 0; q < OPL; ++q)
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[q][s][c] = Acc(0);
-#pragma unroll
-  for (int q = 0; q < (MODE == 2 ? OPL : 1); ++q)
-#pragma unroll
-    for (int s = 0; s < (MODE == 2 ? NS : 1); ++s)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) comp[q][s][c] = 0.0f;
-
-  for (int i = 0; i < g.n_c; ++i) {
-    if (g.ring_fac[i] == 0.0) continue;  // sin^2 theta = 0: B = 0 exactly
-    bool skip;
-    const int t_lo = prepare_ring(g, L, item_id, i, it.t_a, ri, &skip);
-    if (skip || t_lo >= g.n_t) {
-      __syncthreads();
-      continue;
-    }
-    // E_r = exp(i omega r step (1 - cos th)), r < RB, by a double recurrence
-    constexpr int kE = MODE == 0 ? 1 : RB;
-    Acc er[OPL][kE], ei[OPL][kE];
-    if constexpr (MODE != 0) {
-      const double delta = ri.t_step * (1.0 - g.ring_c[i]);
-#pragma unroll
-      for (int q = 0; q < OPL; ++q) {
-        double s1, c1;
-        sincos_f64(om[q] * delta, &s1, &c1);
-        double cr = 1.0, ci = 0.0;
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-          er[q][r] = static_cast<Acc>(cr);
-          ei[q][r] = static_cast<Acc>(ci);
-          const double nr = cr * c1 - ci * s1;
-          const double ni = cr * s1 + ci * c1;
-          cr = nr;
-          ci = ni;
-        }
-      }
-    }
-    for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
-      const int len = min(kTimeChunk, g.n_t - t0);
-      prepare_times<MODE == 2>(g, L, i, t0, len, it, ri);
-      __syncthreads();
-      if constexpr (MODE == 0) {
-        for (int l = 0; l < len; ++l) {
-          const double psi = L.psi[l];
-          const double* v = L.v + l * 2 * NS;
-#pragma unroll
-          for (int q = 0; q < OPL; ++q) {
-            double sn, cs;
-            sincos_f64(om[q] * psi, &sn, &cs);
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-              const double vp = v[2 * s], vx = v[2 * s + 1];
-              acc[q][s][0] = __builtin_fma(vp, cs, acc[q][s][0]);
-              acc[q][s][1] = __builtin_fma(vp, sn, acc[q][s][1]);
-              acc[q][s][2] = __builtin_fma(vx, cs, acc[q][s][2]);
-              acc[q][s][3] = __builtin_fma(vx, sn, acc[q][s][3]);
-            }
-          }
-        }
-      } else {
-        for (int l0 = 0; l0 < len; l0 += RB) {
-          const double psi0 = L.psi[l0];
-          Acc bs[OPL][NS][4];
-#pragma unroll
-          for (int q = 0; q < OPL; ++q)
-#pragma unroll
-            for (int s = 0; s < NS; ++s)
-#pragma unroll
-              for (int c = 0; c < 4; ++c) bs[q][s][c] = Acc(0);
-#pragma unroll
-          for (int r = 0; r < RB; ++r) {
-            const Acc* v;  // zero-padded past len
-            if constexpr (MODE == 2) v = L.vf + (l0 + r) * 2 * NS;
-            else v = L.v + (l0 + r) * 2 * NS;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-              const Acc vp = v[2 * s];
-              const Acc vx = v[2 * s + 1];
-#pragma unroll
-              for (int q = 0; q < OPL; ++q) {
-                bs[q][s][0] = fma(vp, er[q][r], bs[q][s][0]);
-                bs[q][s][1] = fma(vp, ei[q][r], bs[q][s][1]);
-                bs[q][s][2] = fma(vx, er[q][r], bs[q][s][2]);
-                bs[q][s][3] = fma(vx, ei[q][r], bs[q][s][3]);
-              }
-            }
-          }
-#pragma unroll
-          for (int q = 0; q < OPL; ++q) {
-            Acc as, ac;
-            if constexpr (MODE == 2) {
-              sincos_f64_to_f32(om[q] * psi0, &as, &ac);
-            } else {
-              sincos_f64(om[q] * psi0, &as, &ac);
-            }
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-              // (ac + i as) * (b0 + i b1), likewise for the cross polarisation
-              const Acc p0 = fma(ac, bs[q][s][0], -as * bs[q][s][1]);
-              const Acc p1 = fma(ac, bs[q][s][1], as * bs[q][s][0]);
-              const Acc p2 = fma(ac, bs[q][s][2], -as * bs[q][s][3]);
-              const Acc p3 = fma(ac, bs[q][s][3], as * bs[q][s][2]);
-              if constexpr (MODE == 2) {
-                kahan_add(acc[q][s][0], comp[q][s][0], p0);
-                kahan_add(acc[q][s][1], comp[q][s][1], p1);
-                kahan_add(acc[q][s][2], comp[q][s][2], p2);
-                kahan_add(acc[q][s][3], comp[q][s][3], p3);
-              } else {
-                acc[q][s][0] += p0;
-                acc[q][s][1] += p1;
-                acc[q][s][2] += p2;
-                acc[q][s][3] += p3;
-              }
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  double res[OPL][NS][4];
+  spectrum_item<NS, OPL, MODE, RB>(g, item_id, item_id, smem, res);
   // partial[item][pm][s][w] complex
   const int64_t nq = static_cast<int64_t>(g.n_sets_out) * g.n_w;
   double* out = g.partial + static_cast<int64_t>(item_id) * 2 * nq * 2;
@@ -740,21 +66,12 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
     if (wi >= g.n_w) continue;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      double c[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if constexpr (MODE == 2) {
-          c[j] = static_cast<double>(acc[q][s][j]) - static_cast<double>(comp[q][s][j]);
-        } else {
-          c[j] = static_cast<double>(acc[q][s][j]);
-        }
-      }
       const int64_t ip = (static_cast<int64_t>(g.set0 + s) * g.n_w + wi) * 2;
       const int64_t ix = ((nq + static_cast<int64_t>(g.set0 + s) * g.n_w) + wi) * 2;
-      out[ip] = c[0];
-      out[ip + 1] = c[1];
-      out[ix] = c[2];
-      out[ix + 1] = c[3];
+      out[ip] = res[q][s][0];
+      out[ip + 1] = res[q][s][1];
+      out[ix] = res[q][s][2];
+      out[ix + 1] = res[q][s][3];
     }
   }
 }
@@ -835,7 +152,7 @@ __global__ __launch_bounds__(256) void integrand_kernel(SpecArgs g) {
     }
     for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
       const int len = min(kTimeChunk, g.n_t - t0);
-      prepare_times<false>(g, L, i, t0, len, it, ri);
+      prepare_times<false, NS>(g, L, i, t0, len, it, ri);
       __syncthreads();
       for (int l = 0; l < len; ++l) {
         const double psi = L.psi[l];

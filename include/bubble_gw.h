@@ -95,8 +95,8 @@ typedef struct bgw_ensemble bgw_ensemble;
 typedef struct BgwTileConfig {
     size_t block_steps;       /* time steps per factorised phase block: 8 or 16 */
     size_t threads;           /* phase-sum workgroup size: 64, 128 or 256 (lanes own omegas) */
-    size_t items_per_chunk;   /* (realisation, k-hat, bubble) items per chunk of the batch */
-    size_t persistent_blocks; /* workgroups of each persistent launch (0: one per item) */
+    size_t persistent_blocks; /* workgroups of the persistent ensemble launch (0: all that
+                                 fit on the device at once) */
 } BgwTileConfig;
 
 #ifdef __cplusplus

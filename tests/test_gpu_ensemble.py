@@ -1,0 +1,136 @@
+"""Batched ensembles on the GPU: one persistent, graph-captured launch.
+
+Each realisation's spectrum from BubbleEnsemble equals, bit for bit, the
+spectrum of GeneralizedBulkFlow.compute_spectrum on that realisation alone
+(same additions in the same order); FP64 results also match the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gw_fopt_amd import many_bubbles as mb
+from gw_fopt_amd import nucleation
+from gw_fopt_amd.ensemble import BubbleEnsemble
+
+pytestmark = pytest.mark.gpu
+
+N_C, N_PHI, N_T = 12, 24, 60
+
+
+def small_ensemble(counts, seed=21):
+    inter, ext, t_end, boxes = [], [], [], []
+    rng = np.random.default_rng(seed)
+    for r, n in enumerate(counts):
+        beta = rng.uniform(10.0, 30.0)
+        box = float((max(n, 2) * 1.5 * 8.0 * np.pi / (0.68 * beta ** 3)) ** (1.0 / 3.0))
+        b = nucleation.exponential_nucleation(beta, box=box, n_max=n, seed=seed * 1000 + r)
+        inter.append(b)
+        ext.append(nucleation.face_images(b, box))
+        t_end.append(nucleation.percolation_time(b, box=box, n_probe=2048))
+        boxes.append(box)
+    return inter, ext, np.array(t_end), np.array(boxes)
+
+
+def single(interior, exterior, coef, pows):
+    bf = mb.GeneralizedBulkFlow(mb.LatticeBubbles(mb.EmptyLattice(), interior, exterior))
+    bf.set_resolution(N_C, N_PHI)
+    bf.set_gradient_scaling_params(coef, pows)
+    return bf
+
+
+@pytest.fixture(scope="module")
+def ens():
+    counts = [9, 1, 17, 4, 26, 12, 2, 7]
+    inter, ext, t_end, boxes = small_ensemble(counts)
+    w = np.geomspace(0.5, 120.0, 40)[None, :] / boxes[:, None]
+    k = mb.fibonacci_directions(3)
+    return inter, ext, t_end, w, k
+
+
+@pytest.mark.parametrize("precision,tiles", [("fp64", None), ("fp64_blocked", {"block_steps": 8}),
+                                             ("mixed", {"block_steps": 16, "threads": 64})])
+def test_batched_equals_per_realisation(ens, precision, tiles):
+    inter, ext, t_end, w, k = ens
+    coef, pows = [[0.9]], [[5.2]]
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params(coef, pows)
+    out = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision=precision, tiles=tiles)
+    assert out["S"].shape == (len(inter), len(k), 1, w.shape[1])
+    for r in range(len(inter)):
+        bf = single(inter[r], ext[r], coef, pows)
+        if tiles:
+            bf.set_tiles(tiles.get("block_steps", 0), tiles.get("threads", 0))
+        S, _ = bf.compute_spectrum(w[r], k, t_end=t_end[r], n_t=N_T, precision=precision)
+        assert out["S"][r].tobytes() == S.tobytes(), r
+
+
+def test_batched_fp64_matches_oracle(ens):
+    inter, ext, t_end, w, k = ens
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params([[1.0]], [[5.0]])
+    out = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64")
+    for r in (0, 4):
+        So = oracle.spectrum(inter[r], ext[r], k, N_C, N_PHI, [[1.0]], [[5.0]], w[r],
+                             t_end=t_end[r], n_t=N_T)
+        assert np.max(np.abs(out["S"][r] - So) / np.abs(So)) <= 1e-10
+
+
+def test_statistics_and_graph_replays(ens):
+    inter, ext, t_end, w, k = ens
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params([[1.0]], [[5.0]])
+    a = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    # other GPU work between the replays of the captured graph
+    bf = single(inter[2], ext[2], [[1.0]], [[5.0]])
+    bf.compute_spectrum(w[2], k, t_end=t_end[2], n_t=N_T, precision="fp64")
+    b = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    bf.compute_c_integral(w[2], t_end=t_end[2], n_t=N_T)
+    c = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    for key in ("S", "kmean", "mean", "std"):
+        assert a[key].tobytes() == b[key].tobytes() == c[key].tobytes(), key
+    S = a["S"]
+    km = np.zeros(S.shape[:1] + S.shape[2:])
+    for j in range(S.shape[1]):
+        km += S[:, j]
+    km /= S.shape[1]
+    np.testing.assert_array_equal(a["kmean"], km)
+    m = np.zeros(km.shape[1:])
+    for r in range(len(km)):
+        m += km[r]
+    m /= len(km)
+    v = np.zeros_like(m)
+    for r in range(len(km)):
+        v += (km[r] - m) ** 2
+    np.testing.assert_array_equal(a["mean"], m)
+    np.testing.assert_array_equal(a["std"], np.sqrt(v / len(km)))
+
+
+def test_three_sets_two_launches(ens):
+    inter, ext, t_end, w, k = ens
+    coef, pows = [[1.0], [0.5], [1.2]], [[5.0], [4.0], [3.5]]
+    E = BubbleEnsemble(inter[:4], ext[:4])
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params(coef, pows)
+    out = E.compute_spectrum(w[:4], k[:2], t_end=t_end[:4], n_t=N_T, precision="fp64")
+    for r in range(4):
+        bf = single(inter[r], ext[r], coef, pows)
+        S, _ = bf.compute_spectrum(w[r], k[:2], t_end=t_end[r], n_t=N_T, precision="fp64")
+        assert out["S"][r].tobytes() == S.tobytes()
+
+
+def test_autotune_picks_a_valid_shape(ens):
+    inter, ext, t_end, w, k = ens
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params([[1.0]], [[5.0]])
+    best = E.autotune(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    assert best["block_steps"] in (8, 16) and best["threads"] in (64, 128, 256)
+    tuned = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    explicit = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed",
+                                  tiles={"block_steps": best["block_steps"], "threads": best["threads"]})
+    assert tuned["S"].tobytes() == explicit["S"].tobytes()
+    t = E.last_timings()
+    assert t["n_items"] == int(sum(len(b) for b in inter)) * len(k)
