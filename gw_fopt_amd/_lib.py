@@ -45,7 +45,7 @@ class BgwTimings(ctypes.Structure):
 
 class BgwTileConfig(ctypes.Structure):
     _fields_ = [("block_steps", ctypes.c_size_t), ("threads", ctypes.c_size_t),
-                ("persistent_blocks", ctypes.c_size_t)]
+                ("items_per_chunk", ctypes.c_size_t), ("persistent_blocks", ctypes.c_size_t)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -16,6 +16,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <numeric>
@@ -369,7 +370,9 @@ void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
 struct Device {
   int id = -1;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // second branch of the pipelined ensemble graph
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t dep[4] = {nullptr, nullptr, nullptr, nullptr};  // graph dependencies (no timing)
   void init() {
     if (stream) {
       HIPCHECK(hipSetDevice(id));
@@ -380,11 +383,16 @@ struct Device {
       throw CalcError("no HIP device is available: the bulk-flow kernels need a gfx950 GPU");
     HIPCHECK(hipGetDevice(&id));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
     for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    for (auto& e : dep) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   ~Device() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : dep)
+      if (e) (void)hipEventDestroy(e);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -405,7 +413,7 @@ struct Engine {
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
       d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
   // batched ensemble: work units, queue counter, statistics, captured graph
-  DevBuf d_units, d_counter, d_kmean, d_emean, d_estd;
+  DevBuf d_units, d_unit_off, d_counter, d_kmean, d_emean, d_estd;
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
   BgwTimings timings{};
@@ -498,6 +506,7 @@ struct Launch {
   bgw::SpecArgs sa{};
   bgw::RedArgs ra{};
   size_t n_items = 0, n_groups = 0;
+  std::vector<Item> items;               // in (realisation, k-hat, bubble) order
   std::vector<double> item_cost;         // relative phase-sum cost of each item
   std::vector<int32_t> group_off, group_r;
 };
@@ -563,6 +572,7 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   const size_t nq = n_sets * q.n_w;
   Lc.n_items = n_items;
   Lc.n_groups = n_real * n_k;
+  Lc.items = items;
   Lc.item_cost = cost;
   Lc.group_off = group_off;
   Lc.group_r = group_r;
@@ -820,7 +830,7 @@ std::vector<double> run_integrand(Engine& E, const Request& q) {
 struct EnsResult {
   std::vector<double> s, kmean, mean, stdev;
   double ms = 0.0;  // device time of one graph replay
-  BgwTileConfig used{0, 0, 0};
+  BgwTileConfig used{0, 0, 0, 0};
 };
 
 int device_cus(Engine& E) {
@@ -829,8 +839,34 @@ int device_cus(Engine& E) {
   return n;
 }
 
-// units: groups in launch order (all groups, largest first, when null).
-// replays >= 1 graph launches; the timing is the mean over them.
+// Work units, (realisation, k-hat) groups, most expensive first.
+std::vector<int32_t> unit_order(const Engine& E, const Launch& Lc) {
+  std::vector<double> gcost(Lc.n_groups, 0.0);
+  for (size_t g = 0; g < Lc.n_groups; ++g) {
+    const size_t r = static_cast<size_t>(Lc.group_r[g]);
+    for (int32_t it = Lc.group_off[g]; it < Lc.group_off[g + 1]; ++it)
+      gcost[g] += Lc.item_cost[it] + 1e-4 * static_cast<double>(E.real[r].n_tot);
+  }
+  std::vector<int32_t> order(Lc.n_groups);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return gcost[x] > gcost[y]; });
+  return order;
+}
+
+// Launch arguments of one chunk of the pipelined form.
+struct EnsChunk {
+  int item0 = 0, n_items = 0;  // items [item0, item0 + n_items) of the unit-ordered list
+  int u0 = 0, n_units = 0;     // units [u0, u0 + n_units)
+  int buf = 0;                 // tau / perm buffer (0 or 1)
+};
+
+// units: the units to run, in launch order (null: every unit, largest first).
+// tiles.items_per_chunk == 0: one fused persistent launch per set chunk;
+// otherwise chunks of units, geometry on one stream and phase sums + reduce on
+// the other, the delay rows double-buffered.  Either way the whole sequence
+// is one captured hipGraph, re-captured only when the launch shape, the sizes
+// or a device buffer changes.  replays >= 1 graph launches; ms is their mean.
 EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
                        const std::vector<int32_t>* units, bool want_outputs, int replays) {
   const Resolution& res = *q.res;
@@ -839,6 +875,7 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   const size_t n_real = E.real.size(), n_k = q.k.size();
   EnsResult out;
   const size_t nq = n_sets * q.n_w;
+  const bool pipelined = tiles.items_per_chunk > 0;
   Request qq = q;
   qq.item_buffers = false;
   qq.tile_threads = static_cast<int>(tiles.threads);
@@ -847,89 +884,205 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   setup(E, qq, Lc);
   hipStream_t st = E.dev.stream;
   const size_t n_dir = res.n_c * res.n_phi;
+  const std::vector<int32_t> order = units ? *units : unit_order(E, Lc);
 
-  // work units: (realisation, k-hat) groups, most expensive first
-  std::vector<int32_t> order;
-  if (units) {
-    order = *units;
-  } else {
-    std::vector<double> gcost(Lc.n_groups, 0.0);
-    for (size_t g = 0; g < Lc.n_groups; ++g) {
-      const size_t r = static_cast<size_t>(Lc.group_r[g]);
-      for (int32_t it = Lc.group_off[g]; it < Lc.group_off[g + 1]; ++it)
-        gcost[g] += Lc.item_cost[it] + 1e-4 * static_cast<double>(E.real[r].n_tot);
-    }
-    order.resize(Lc.n_groups);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return gcost[x] > gcost[y]; });
-  }
-  upload(E.d_units, order, st);
   E.d_counter.ensure(256);
   E.d_c.ensure(std::max<size_t>(1, Lc.n_groups * 2 * nq) * 2 * sizeof(double));
   E.d_s.ensure(std::max<size_t>(1, Lc.n_groups * nq) * sizeof(double));
   E.d_kmean.ensure(std::max<size_t>(1, n_real * nq) * sizeof(double));
   E.d_emean.ensure(std::max<size_t>(1, nq) * sizeof(double));
   E.d_estd.ensure(std::max<size_t>(1, nq) * sizeof(double));
+  upload(E.d_units, order, st);
 
-  bgw::EnsArgs ea{};
-  ea.geo = Lc.ga;
-  ea.geo.first_out = nullptr;
-  ea.geo.dtab_out = nullptr;
-  ea.spec = Lc.sa;
-  ea.unit_group = E.d_units.as<int32_t>();
-  ea.group_item_off = E.d_group_off.as<int32_t>();
-  ea.group_r = E.d_group_r.as<int32_t>();
-  ea.n_units = static_cast<int>(order.size());
-  ea.counter = E.d_counter.as<int>();
-  ea.c_out = E.d_c.as<double>();
-  ea.s_out = E.d_s.as<double>();
-
-  // launch shape: one launch per chunk of <= kEnsembleMaxSets sets
-  std::vector<bgw::EnsArgs> launches;
-  int n_blocks = static_cast<int>(tiles.persistent_blocks);
-  for (size_t s0 = 0; s0 < n_sets; s0 += bgw::kEnsembleMaxSets) {
-    bgw::EnsArgs eb = ea;
-    eb.spec.set0 = static_cast<int>(s0);
-    eb.spec.n_sets = static_cast<int>(std::min<size_t>(bgw::kEnsembleMaxSets, n_sets - s0));
-    eb.spec.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
-    eb.spec.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
-    eb.spec.time_tables = 0;
-    eb.spec.time_tables = bgw::spectrum_time_tables(eb.spec) ? 1 : 0;
-    if (bgw::ensemble_lds_bytes(eb, q.mode) + 24 * 1024 > 160 * 1024)
-      throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
-    const int per_cu = bgw::ensemble_blocks_per_cu(eb, q.mode);  // also sets the LDS attribute
-    if (per_cu <= 0) throw CalcError("the ensemble kernel cannot be launched with these tiles");
-    if (tiles.persistent_blocks == 0) n_blocks = std::max(n_blocks, per_cu * device_cus(E));
-    launches.push_back(eb);
-  }
-  n_blocks = std::max(n_blocks, 1);
-  E.d_tau.ensure(static_cast<size_t>(n_blocks) * n_dir * sizeof(double));
-  E.d_perm.ensure(static_cast<size_t>(n_blocks) * n_dir * sizeof(uint16_t));
-  for (auto& eb : launches) {
-    eb.geo.tau = E.d_tau.as<double>();
-    eb.geo.perm = E.d_perm.as<uint16_t>();
-    eb.spec.tau = eb.geo.tau;
-    eb.spec.perm = eb.geo.perm;
-  }
-  out.used = tiles;
-  out.used.persistent_blocks = static_cast<size_t>(n_blocks);
-  out.used.block_steps = static_cast<size_t>(qq.tile_rb);
-
-  // graph: [queue reset, persistent kernel] per set chunk, then the statistics
   std::ostringstream key;
-  key << q.mode << ':' << n_blocks << ':' << order.size() << ':' << n_real << ':' << n_k << ':'
-      << n_sets << ':' << q.n_w << ':' << q.n_t << ':' << qq.tile_threads << ':' << qq.tile_rb;
-  for (const auto& eb : launches)
-    key << ':' << eb.spec.time_tables << ':' << static_cast<const void*>(eb.geo.tau) << ':'
-        << static_cast<const void*>(eb.spec.real) << ':' << static_cast<const void*>(eb.spec.items)
-        << ':' << static_cast<const void*>(eb.spec.omega) << ':'
-        << static_cast<const void*>(eb.spec.coef) << ':'
-        << static_cast<const void*>(eb.unit_group) << ':'
-        << static_cast<const void*>(eb.group_item_off) << ':'
-        << static_cast<const void*>(eb.c_out) << ':' << static_cast<const void*>(eb.s_out) << ':'
-        << static_cast<const void*>(eb.geo.cand) << ':' << static_cast<const void*>(eb.geo.rot);
-  key << ':' << static_cast<const void*>(E.d_kmean.p) << ':' << static_cast<const void*>(E.d_emean.p);
+  key << (pipelined ? 'P' : 'F') << ':' << q.mode << ':' << order.size() << ':' << n_real << ':'
+      << n_k << ':' << n_sets << ':' << q.n_w << ':' << q.n_t << ':' << qq.tile_threads << ':'
+      << qq.tile_rb << ':' << tiles.items_per_chunk << ':' << tiles.persistent_blocks;
+  std::vector<std::function<hipError_t()>> body;  // graph body, recorded under capture
+  int n_blocks = 0;
+
+  if (!pipelined) {
+    bgw::EnsArgs ea{};
+    ea.geo = Lc.ga;
+    ea.geo.first_out = nullptr;
+    ea.geo.dtab_out = nullptr;
+    ea.spec = Lc.sa;
+    ea.unit_group = E.d_units.as<int32_t>();
+    ea.group_item_off = E.d_group_off.as<int32_t>();
+    ea.group_r = E.d_group_r.as<int32_t>();
+    ea.n_units = static_cast<int>(order.size());
+    ea.counter = E.d_counter.as<int>();
+    ea.c_out = E.d_c.as<double>();
+    ea.s_out = E.d_s.as<double>();
+    std::vector<bgw::EnsArgs> launches;
+    n_blocks = static_cast<int>(tiles.persistent_blocks);
+    for (size_t s0 = 0; s0 < n_sets; s0 += bgw::kEnsembleMaxSets) {
+      bgw::EnsArgs eb = ea;
+      eb.spec.set0 = static_cast<int>(s0);
+      eb.spec.n_sets = static_cast<int>(std::min<size_t>(bgw::kEnsembleMaxSets, n_sets - s0));
+      eb.spec.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
+      eb.spec.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
+      eb.spec.time_tables = 0;
+      eb.spec.time_tables = bgw::spectrum_time_tables(eb.spec) ? 1 : 0;
+      if (bgw::ensemble_lds_bytes(eb, q.mode) + 24 * 1024 > 160 * 1024)
+        throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+      const int per_cu = bgw::ensemble_blocks_per_cu(eb, q.mode);  // sets the LDS limit too
+      if (per_cu <= 0) throw CalcError("the ensemble kernel cannot be launched with these tiles");
+      if (tiles.persistent_blocks == 0) n_blocks = std::max(n_blocks, per_cu * device_cus(E));
+      launches.push_back(eb);
+    }
+    n_blocks = std::max(n_blocks, 1);
+    E.d_tau.ensure(static_cast<size_t>(n_blocks) * n_dir * sizeof(double));
+    E.d_perm.ensure(static_cast<size_t>(n_blocks) * n_dir * sizeof(uint16_t));
+    for (auto& eb : launches) {
+      eb.geo.tau = E.d_tau.as<double>();
+      eb.geo.perm = E.d_perm.as<uint16_t>();
+      eb.spec.tau = eb.geo.tau;
+      eb.spec.perm = eb.geo.perm;
+      body.push_back([eb, &q, n_blocks, st]() { return bgw::launch_ensemble(eb, q.mode, n_blocks, st); });
+    }
+    key << ':' << n_blocks;
+  } else {
+    // items in unit order; chunks of whole units with about items_per_chunk items
+    std::vector<Item> items_u;
+    std::vector<int32_t> unit_off(1, 0), spec_order;
+    const std::vector<Item>& items_orig = Lc.items;
+    items_u.reserve(Lc.n_items);
+    std::vector<double> cost_u;
+    cost_u.reserve(Lc.n_items);
+    for (int32_t g : order) {
+      for (int32_t it = Lc.group_off[g]; it < Lc.group_off[g + 1]; ++it) {
+        items_u.push_back(items_orig[it]);
+        cost_u.push_back(Lc.item_cost[it]);
+      }
+      unit_off.push_back(static_cast<int32_t>(items_u.size()));
+    }
+    std::vector<EnsChunk> chunks;
+    size_t max_chunk = 0;
+    for (size_t u = 0; u < order.size();) {
+      EnsChunk c;
+      c.u0 = static_cast<int>(u);
+      c.item0 = unit_off[u];
+      while (u < order.size() &&
+             (c.n_units == 0 ||
+              static_cast<size_t>(unit_off[u + 1] - c.item0) <= tiles.items_per_chunk)) {
+        ++u;
+        ++c.n_units;
+      }
+      c.n_items = unit_off[u] - c.item0;
+      c.buf = static_cast<int>(chunks.size() & 1);
+      max_chunk = std::max(max_chunk, static_cast<size_t>(c.n_items));
+      chunks.push_back(c);
+    }
+    // phase-sum launch order inside each chunk: most expensive items first
+    spec_order.resize(items_u.size());
+    for (const auto& c : chunks) {
+      std::iota(spec_order.begin() + c.item0, spec_order.begin() + c.item0 + c.n_items, c.item0);
+      std::stable_sort(spec_order.begin() + c.item0, spec_order.begin() + c.item0 + c.n_items,
+                       [&](int32_t x, int32_t y) { return cost_u[x] > cost_u[y]; });
+    }
+    upload(E.d_items, items_u, st);
+    upload(E.d_unit_off, unit_off, st);
+    upload(E.d_order, spec_order, st);
+    const size_t rows = std::max<size_t>(1, max_chunk);
+    E.d_tau.ensure(2 * rows * n_dir * sizeof(double));
+    E.d_perm.ensure(2 * rows * n_dir * sizeof(uint16_t));
+    E.d_partial.ensure(rows * 2 * nq * 2 * sizeof(double));
+    bgw::GeoArgs ga = Lc.ga;
+    ga.items = E.d_items.as<Item>();
+    ga.first_out = nullptr;
+    ga.dtab_out = nullptr;
+    bgw::SpecArgs sa = Lc.sa;
+    sa.items = ga.items;
+    sa.partial = E.d_partial.as<double>();
+    bgw::RedArgs ra = Lc.ra;
+    ra.partial = sa.partial;
+    ra.unit_group = E.d_units.as<int32_t>();
+    ra.unit_off = E.d_unit_off.as<int32_t>();
+    ra.c_out = E.d_c.as<double>();
+    ra.s_out = E.d_s.as<double>();
+    for (size_t s0 = 0; s0 < n_sets; s0 += kMaxSetsPerLaunch) {
+      bgw::SpecArgs sb = sa;
+      sb.n_sets = static_cast<int>(std::min(kMaxSetsPerLaunch, n_sets - s0));
+      sb.time_tables = 0;
+      sb.time_tables = bgw::spectrum_time_tables(sb) ? 1 : 0;
+      if (bgw::spectrum_lds_bytes(sb, q.mode) > 160 * 1024)
+        throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+    }
+    hipStream_t s2 = E.dev.stream2;
+    hipEvent_t fork = E.dev.dep[0], geo_done = E.dev.dep[1];
+    hipEvent_t spec_done[2] = {E.dev.dep[2], E.dev.dep[3]};
+    double* tau0 = E.d_tau.as<double>();
+    uint16_t* perm0 = E.d_perm.as<uint16_t>();
+    // stream st: geometry of chunk c (waits for the phase sums of chunk c - 2,
+    // the previous user of its buffer); stream s2: phase sums and reduce.
+    body.push_back([=]() {
+      hipError_t e = hipEventRecord(fork, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s2, fork, 0);
+      return e;
+    });
+    for (size_t ci = 0; ci < chunks.size(); ++ci) {
+      const EnsChunk c = chunks[ci];
+      bgw::GeoArgs gc = ga;
+      gc.item0 = c.item0;
+      gc.tau = tau0 + static_cast<size_t>(c.buf) * rows * n_dir;
+      gc.perm = perm0 + static_cast<size_t>(c.buf) * rows * n_dir;
+      body.push_back([=]() {
+        hipError_t e = hipSuccess;
+        if (ci >= 2) e = hipStreamWaitEvent(st, spec_done[c.buf], 0);
+        if (e == hipSuccess) e = bgw::launch_geometry(gc, c.n_items, st);
+        if (e == hipSuccess) e = hipEventRecord(geo_done, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s2, geo_done, 0);
+        return e;
+      });
+      for (size_t s0 = 0; s0 < n_sets; s0 += kMaxSetsPerLaunch) {
+        bgw::SpecArgs sb = sa;
+        sb.item0 = c.item0;
+        sb.order = E.d_order.as<int32_t>() + c.item0;
+        sb.tau = gc.tau;
+        sb.perm = gc.perm;
+        sb.set0 = static_cast<int>(s0);
+        sb.n_sets = static_cast<int>(std::min(kMaxSetsPerLaunch, n_sets - s0));
+        sb.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
+        sb.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
+        sb.time_tables = 0;
+        sb.time_tables = bgw::spectrum_time_tables(sb) ? 1 : 0;
+        const int mode = q.mode;
+        body.push_back([=]() { return bgw::launch_spectrum(sb, c.n_items, mode, s2); });
+      }
+      bgw::RedArgs rc = ra;
+      rc.u0 = c.u0;
+      rc.item0 = c.item0;
+      body.push_back([=]() {
+        hipError_t e = bgw::launch_reduce(rc, c.n_units, s2);
+        if (e == hipSuccess) e = hipEventRecord(spec_done[c.buf], s2);
+        return e;
+      });
+    }
+    key << ':' << chunks.size() << ':' << rows << ':' << static_cast<const void*>(E.d_unit_off.p)
+        << ':' << static_cast<const void*>(E.d_partial.p) << ':'
+        << static_cast<const void*>(E.d_order.p);
+  }
+  // statistics on the stream that ran the phase sums, then join
+  hipStream_t s_last = pipelined ? E.dev.stream2 : st;
+  body.push_back([&, s_last]() {
+    hipError_t e = bgw::launch_ensemble_stats(
+        E.d_s.as<double>(), static_cast<int>(n_real), static_cast<int>(n_k), static_cast<int>(nq),
+        E.d_kmean.as<double>(), E.d_emean.as<double>(), E.d_estd.as<double>(), s_last);
+    if (e == hipSuccess && s_last != st) {
+      e = hipEventRecord(E.dev.dep[0], s_last);
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, E.dev.dep[0], 0);
+    }
+    return e;
+  });
+  key << ':' << static_cast<const void*>(E.d_tau.p) << ':' << static_cast<const void*>(E.d_perm.p)
+      << ':' << static_cast<const void*>(E.d_items.p) << ':' << static_cast<const void*>(E.d_real.p)
+      << ':' << static_cast<const void*>(E.d_omega.p) << ':' << static_cast<const void*>(E.d_coef.p)
+      << ':' << static_cast<const void*>(E.d_units.p) << ':'
+      << static_cast<const void*>(E.d_group_off.p) << ':' << static_cast<const void*>(E.d_c.p)
+      << ':' << static_cast<const void*>(E.d_s.p) << ':' << static_cast<const void*>(E.d_cand.p)
+      << ':' << static_cast<const void*>(E.d_rot.p) << ':' << static_cast<const void*>(E.d_kmean.p)
+      << ':' << static_cast<const void*>(E.d_emean.p);
   if (!E.graph || E.graph_key != key.str()) {
     if (E.graph) {
       HIPCHECK(hipGraphExecDestroy(E.graph));
@@ -939,14 +1092,8 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
     hipGraph_t graph = nullptr;
     HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t err = hipSuccess;
-    for (const auto& eb : launches) {
-      if (err == hipSuccess) err = bgw::launch_ensemble(eb, q.mode, n_blocks, st);
-    }
-    if (err == hipSuccess)
-      err = bgw::launch_ensemble_stats(E.d_s.as<double>(), static_cast<int>(n_real),
-                                       static_cast<int>(n_k), static_cast<int>(nq),
-                                       E.d_kmean.as<double>(), E.d_emean.as<double>(),
-                                       E.d_estd.as<double>(), st);
+    for (auto& f : body)
+      if (err == hipSuccess) err = f();
     hipError_t end = hipStreamEndCapture(st, &graph);
     HIPCHECK(err);
     HIPCHECK(end);
@@ -974,6 +1121,9 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   }
   HIPCHECK(hipStreamSynchronize(st));
   out.ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]) / std::max(1, replays);
+  out.used = tiles;
+  out.used.block_steps = static_cast<size_t>(qq.tile_rb);
+  out.used.persistent_blocks = pipelined ? 0 : static_cast<size_t>(n_blocks);
   E.timings.geometry_ms = 0.0;
   E.timings.spectrum_ms = out.ms;
   E.timings.reduce_ms = 0.0;
@@ -1050,7 +1200,7 @@ struct bgw_generalized_bulk_flow {
   Resolution res;
   Scaling sc;
   size_t num_threads = 0;
-  BgwTileConfig tiles{0, 0, 0};
+  BgwTileConfig tiles{0, 0, 0, 0};
   Engine engine;
 };
 
@@ -1059,7 +1209,7 @@ struct bgw_ensemble {
   Resolution res;
   Scaling sc;
   Engine engine;
-  BgwTileConfig tiles{0, 0, 0};
+  BgwTileConfig tiles{0, 0, 0, 0};
   bool tuned = false;
 };
 
@@ -1071,6 +1221,8 @@ void check_tiles(const BgwTileConfig& t) {
   if (t.threads != 0 && t.threads != 64 && t.threads != 128 && t.threads != 256)
     throw CalcError("tiles.threads must be 0, 64, 128 or 256");
   if (t.persistent_blocks > 65536) throw CalcError("tiles.persistent_blocks must be <= 65536");
+  if (t.items_per_chunk > (size_t(1) << 24))
+    throw CalcError("tiles.items_per_chunk must be <= 16777216");
 }
 
 void apply_tiles(const BgwTileConfig& t, Request& q) {
@@ -1552,7 +1704,7 @@ BgwStatus bgw_generalized_bulk_flow_set_tiles(bgw_generalized_bulk_flow* bulk_fl
                                               const BgwTileConfig* tiles) {
   return run_ffi([&] {
     auto* h = bulk_flow_handle(bulk_flow);
-    BgwTileConfig t{0, 0, 0};
+    BgwTileConfig t{0, 0, 0, 0};
     if (tiles) t = *tiles;
     check_tiles(t);
     h->tiles = t;
@@ -1736,19 +1888,22 @@ BgwStatus bgw_ensemble_autotune(bgw_ensemble* ensemble, const double* w_arr, siz
       const size_t m = std::max<size_t>(1, order.size() / 4096);
       for (size_t i = 0; i < order.size(); i += m) sample.push_back(order[i]);
     }
+    // chunking: 0 = fused persistent kernel; else pipelined chunks of ~N items
+    const std::vector<size_t> chunking = {0, 16384, 65536};
     double best_ms = std::numeric_limits<double>::infinity();
     BgwTileConfig win = ensemble->tiles;
-    for (size_t t : threads)
-      for (size_t rb : blocks) {
-        BgwTileConfig cand{rb, t, 0};
-        (void)run_ensemble(ensemble->engine, q, cand, &sample, false, 1);  // capture + warm-up
-        EnsResult r = run_ensemble(ensemble->engine, q, cand, &sample, false, 2);
-        if (r.ms < best_ms) {
-          best_ms = r.ms;
-          win = r.used;
-          win.persistent_blocks = 0;
+    for (size_t ch : chunking)
+      for (size_t t : threads)
+        for (size_t rb : blocks) {
+          BgwTileConfig cand{rb, t, ch, 0};
+          (void)run_ensemble(ensemble->engine, q, cand, &sample, false, 1);  // capture + warm-up
+          EnsResult r = run_ensemble(ensemble->engine, q, cand, &sample, false, 2);
+          if (r.ms < best_ms) {
+            best_ms = r.ms;
+            win = r.used;
+            win.persistent_blocks = 0;
+          }
         }
-      }
     ensemble->tiles = win;
     ensemble->tuned = true;
     if (best) *best = win;

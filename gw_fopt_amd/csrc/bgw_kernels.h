@@ -25,6 +25,7 @@ struct GeoArgs {
   uint16_t* perm;         // (n_items, n_c * n_phi), phi index of each sorted entry
   int32_t* first_out;     // optional (n_items, n_c * n_phi), unsorted
   double* dtab_out;       // optional (n_items, n_c * n_phi), unsorted
+  int item0;              // workgroup b works on item item0 + b, tau / perm row b
 };
 
 struct SpecArgs {
@@ -53,6 +54,7 @@ struct SpecArgs {
   int threads;            // workgroup size (0: automatic); lanes own omegas
   int time_tables;        // per-item time tables in LDS (spectrum_time_tables)
   double* partial;        // (n_items, 2, n_sets_out, n_w) complex
+  int item0;              // tau / perm / partial rows are item - item0
   // adjoint only
   const double* grad_c;   // (n_groups, 2, n_sets, n_w) complex cotangent of C
   double* grad_out;       // (n_items, 2, n_sets_out, n_terms): d/dcoef, d/dpow
@@ -67,6 +69,12 @@ struct RedArgs {
   int n_sets, n_w;
   double* c_out;             // (n_groups, 2, n_sets, n_w) complex
   double* s_out;             // (n_groups, n_sets, n_w)
+  // unit mode (unit_group != null): workgroup b reduces unit u0 + b, i.e. group
+  // unit_group[u0 + b] with items [unit_off[u0 + b], unit_off[u0 + b + 1]);
+  // partial rows are item - item0
+  const int32_t* unit_group;
+  const int32_t* unit_off;
+  int u0, item0;
 };
 
 // Batched ensemble: one persistent launch over (realisation, k-hat) groups.

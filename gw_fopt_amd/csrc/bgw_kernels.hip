@@ -42,10 +42,9 @@ __global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
   __shared__ RotCand cand_lds[kGeoLdsCand];
   __shared__ double sort_key[kSortCap];
   __shared__ int32_t sort_val[kSortCap];
-  const int item_id = blockIdx.x;
-  const int64_t n_dir = static_cast<int64_t>(g.n_c) * g.n_phi;
-  geometry_item(g, item_id, g.tau + item_id * n_dir, g.perm + item_id * n_dir, cand_lds, sort_key,
-                sort_val);
+  const int item_id = g.item0 + static_cast<int>(blockIdx.x);
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * g.n_c * g.n_phi;
+  geometry_item(g, item_id, g.tau + row, g.perm + row, cand_lds, sort_key, sort_val);
 }
 
 // ---------------------------------------------------------------------------
@@ -54,12 +53,13 @@ __global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
 template <int NS, int OPL, int MODE, int RB>
 __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
   extern __shared__ double smem[];
-  const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
+  const int item_id = g.order ? g.order[blockIdx.x] : g.item0 + static_cast<int>(blockIdx.x);
+  const int64_t row = item_id - g.item0;
   double res[OPL][NS][4];
-  spectrum_item<NS, OPL, MODE, RB>(g, item_id, item_id, smem, res);
+  spectrum_item<NS, OPL, MODE, RB>(g, item_id, row, smem, res);
   // partial[item][pm][s][w] complex
   const int64_t nq = static_cast<int64_t>(g.n_sets_out) * g.n_w;
-  double* out = g.partial + static_cast<int64_t>(item_id) * 2 * nq * 2;
+  double* out = g.partial + row * 2 * nq * 2;
 #pragma unroll
   for (int q = 0; q < OPL; ++q) {
     const int wi = threadIdx.x + q * blockDim.x;
@@ -80,15 +80,24 @@ __global__ __launch_bounds__(256) void spectrum_kernel(SpecArgs g) {
 // Stage 3: deterministic sum over reference bubbles, spectrum
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void reduce_kernel(RedArgs g) {
-  const int grp = blockIdx.x;
-  const int i0 = g.group_off[grp], i1 = g.group_off[grp + 1];
+  int grp, i0, i1;
+  if (g.unit_group) {
+    const int u = g.u0 + static_cast<int>(blockIdx.x);
+    grp = g.unit_group[u];
+    i0 = g.unit_off[u];
+    i1 = g.unit_off[u + 1];
+  } else {
+    grp = blockIdx.x;
+    i0 = g.group_off[grp];
+    i1 = g.group_off[grp + 1];
+  }
   const int64_t nq = static_cast<int64_t>(g.n_sets) * g.n_w;
   const RealInfo ri = g.real[g.group_r[grp]];
   const double eight_pi = 8.0 * 3.14159265358979323846;
   for (int64_t q = threadIdx.x; q < nq; q += blockDim.x) {
     double pr = 0.0, pi = 0.0, xr = 0.0, xi = 0.0;
     for (int item = i0; item < i1; ++item) {
-      const double* p = g.partial + static_cast<int64_t>(item) * 2 * nq * 2;
+      const double* p = g.partial + static_cast<int64_t>(item - g.item0) * 2 * nq * 2;
       pr += p[q * 2];
       pi += p[q * 2 + 1];
       xr += p[(nq + q) * 2];

@@ -90,7 +90,7 @@ class BubbleEnsemble:
         "mean": (n_sets, n_w), "std": (n_sets, n_w)}.  ``w_arr``: (n_w,) shared or
         (n_real, n_w) per realisation; ``t_begin`` / ``t_end``: scalars or (n_real,);
         ``tiles``: None (autotuned or automatic) or dict(block_steps, threads,
-        persistent_blocks)."""
+        items_per_chunk, persistent_blocks)."""
         w, k, tb, te = self._args(w_arr, k_hat, t_begin, t_end)
         n_w = w.shape[1]
         S = np.empty((self.n_real, len(k), self._n_sets, n_w))
@@ -100,6 +100,7 @@ class BubbleEnsemble:
         t = None
         if tiles is not None:
             t = _lib.BgwTileConfig(int(tiles.get("block_steps", 0)), int(tiles.get("threads", 0)),
+                                   int(tiles.get("items_per_chunk", 0)),
                                    int(tiles.get("persistent_blocks", 0)))
         self._call(_lib.lib().bgw_ensemble_compute_spectrum, _lib.dptr(w), n_w, _lib.dptr(k),
                    len(k), _lib.dptr(tb), _lib.dptr(te), int(n_t), _precision(precision),

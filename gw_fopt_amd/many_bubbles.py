@@ -487,7 +487,7 @@ class GeneralizedBulkFlow:
 
     def set_tiles(self, block_steps: int = 0, threads: int = 0) -> None:
         """Launch shape of the phase-sum kernel (0: automatic)."""
-        t = _lib.BgwTileConfig(int(block_steps), int(threads), 0)
+        t = _lib.BgwTileConfig(int(block_steps), int(threads), 0, 0)
         self._call(_lib.lib().bgw_generalized_bulk_flow_set_tiles, ctypes.byref(t))
 
     def last_timings(self) -> dict:

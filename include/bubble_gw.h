@@ -95,8 +95,12 @@ typedef struct bgw_ensemble bgw_ensemble;
 typedef struct BgwTileConfig {
     size_t block_steps;       /* time steps per factorised phase block: 8 or 16 */
     size_t threads;           /* phase-sum workgroup size: 64, 128 or 256 (lanes own omegas) */
-    size_t persistent_blocks; /* workgroups of the persistent ensemble launch (0: all that
-                                 fit on the device at once) */
+    size_t items_per_chunk;   /* ensembles: 0 = one fused persistent launch (geometry and
+                                 phase sums per (realisation, k-hat) unit); N > 0 = chunks of
+                                 about N (realisation, k-hat, bubble) items whose geometry and
+                                 phase-sum kernels run pipelined on two streams */
+    size_t persistent_blocks; /* workgroups of the persistent launch (0: all that fit on the
+                                 device at once) */
 } BgwTileConfig;
 
 #ifdef __cplusplus

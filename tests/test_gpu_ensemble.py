@@ -47,8 +47,15 @@ def ens():
     return inter, ext, t_end, w, k
 
 
-@pytest.mark.parametrize("precision,tiles", [("fp64", None), ("fp64_blocked", {"block_steps": 8}),
-                                             ("mixed", {"block_steps": 16, "threads": 64})])
+@pytest.mark.parametrize("precision,tiles", [
+    ("fp64", None),
+    ("fp64_blocked", {"block_steps": 8}),
+    ("mixed", {"block_steps": 16, "threads": 64}),
+    # pipelined chunks (geometry and phase sums on two streams, double-buffered
+    # delay rows): several chunks, some units larger than a chunk
+    ("fp64", {"items_per_chunk": 25}),
+    ("mixed", {"block_steps": 8, "threads": 64, "items_per_chunk": 40}),
+])
 def test_batched_equals_per_realisation(ens, precision, tiles):
     inter, ext, t_end, w, k = ens
     coef, pows = [[0.9]], [[5.2]]
@@ -82,13 +89,16 @@ def test_statistics_and_graph_replays(ens):
     E = BubbleEnsemble(inter, ext)
     E.set_resolution(N_C, N_PHI)
     E.set_gradient_scaling_params([[1.0]], [[5.0]])
-    a = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    tl = {"items_per_chunk": 33}
+    a = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed", tiles=tl)
     # other GPU work between the replays of the captured graph
     bf = single(inter[2], ext[2], [[1.0]], [[5.0]])
     bf.compute_spectrum(w[2], k, t_end=t_end[2], n_t=N_T, precision="fp64")
-    b = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    b = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed", tiles=tl)
     bf.compute_c_integral(w[2], t_end=t_end[2], n_t=N_T)
-    c = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    c = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed", tiles=tl)
+    d = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")  # fused form
+    assert d["S"].tobytes() == a["S"].tobytes()
     for key in ("S", "kmean", "mean", "std"):
         assert a[key].tobytes() == b[key].tobytes() == c[key].tobytes(), key
     S = a["S"]
@@ -108,13 +118,17 @@ def test_statistics_and_graph_replays(ens):
     np.testing.assert_array_equal(a["std"], np.sqrt(v / len(km)))
 
 
-def test_three_sets_two_launches(ens):
+@pytest.mark.parametrize("chunk", [0, 30])
+def test_three_sets(ens, chunk):
+    """Three parameter sets: two fused launches (sets 0-1, then 2), or one
+    phase-sum launch per pipelined chunk."""
     inter, ext, t_end, w, k = ens
     coef, pows = [[1.0], [0.5], [1.2]], [[5.0], [4.0], [3.5]]
     E = BubbleEnsemble(inter[:4], ext[:4])
     E.set_resolution(N_C, N_PHI)
     E.set_gradient_scaling_params(coef, pows)
-    out = E.compute_spectrum(w[:4], k[:2], t_end=t_end[:4], n_t=N_T, precision="fp64")
+    out = E.compute_spectrum(w[:4], k[:2], t_end=t_end[:4], n_t=N_T, precision="fp64",
+                             tiles={"items_per_chunk": chunk})
     for r in range(4):
         bf = single(inter[r], ext[r], coef, pows)
         S, _ = bf.compute_spectrum(w[r], k[:2], t_end=t_end[r], n_t=N_T, precision="fp64")
@@ -128,9 +142,11 @@ def test_autotune_picks_a_valid_shape(ens):
     E.set_gradient_scaling_params([[1.0]], [[5.0]])
     best = E.autotune(w, k, t_end=t_end, n_t=N_T, precision="mixed")
     assert best["block_steps"] in (8, 16) and best["threads"] in (64, 128, 256)
+    assert best["items_per_chunk"] in (0, 16384, 65536)
     tuned = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
     explicit = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed",
-                                  tiles={"block_steps": best["block_steps"], "threads": best["threads"]})
+                                  tiles={"block_steps": best["block_steps"], "threads": best["threads"],
+                                         "items_per_chunk": best["items_per_chunk"]})
     assert tuned["S"].tobytes() == explicit["S"].tobytes()
     t = E.last_timings()
     assert t["n_items"] == int(sum(len(b) for b in inter)) * len(k)
