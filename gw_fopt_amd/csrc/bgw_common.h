@@ -29,7 +29,7 @@ struct alignas(16) CandEntry {
   double key;        // collision-delay lower bound, or -inf
   int32_t idx;       // total bubble index b (interior first, then exterior)
   int32_t flags;     // bit 0: regular pair
-  double scale;      // |d_t| + |d_spatial|
+  double len;        // |d_spatial|
 };
 static_assert(sizeof(CandEntry) == 64, "CandEntry must be 64 bytes");
 

@@ -137,19 +137,15 @@ EnsKernel pick_opl(int opl, int mode, int rb) {
 // The kernel instance and workgroup size for a launch shape; null if invalid.
 EnsKernel ensemble_instance(const EnsArgs& a, int mode, int* threads_out) {
   if (mode < 0 || mode > 2) return nullptr;
-  int threads = a.spec.threads;
-  if (threads <= 0) {
-    threads = ((a.spec.n_w + 63) / 64) * 64;
-    if (threads > 256) threads = 256;
-  }
+  const int threads = resolve_threads(a.spec.threads, a.spec.n_w, mode);
   if (threads < 64 || threads > 256 || threads % 64) return nullptr;
   const int opl_needed = (a.spec.n_w + threads - 1) / threads;
   if (opl_needed > 4) return nullptr;
   const int opl = opl_needed <= 1 ? 1 : (opl_needed <= 2 ? 2 : 4);
   *threads_out = threads;
   switch (a.spec.n_sets) {
-    case 1: return pick_opl<1>(opl, mode, a.spec.mixed_block);
-    case 2: return pick_opl<2>(opl, mode, a.spec.mixed_block);
+    case 1: return pick_opl<1>(opl, mode, resolve_block_steps(a.spec.mixed_block, mode));
+    case 2: return pick_opl<2>(opl, mode, resolve_block_steps(a.spec.mixed_block, mode));
     default: return nullptr;
   }
 }

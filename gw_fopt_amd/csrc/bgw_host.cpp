@@ -203,7 +203,7 @@ void build_candidates(const Bubbles& B, size_t a, CandEntry* out, int32_t* inv) 
     for (int k = 0; k < 4; ++k) e.d[k] = xb[k] - xa[k];
     e.dsq = lscalar(e.d, e.d);
     const double ns = std::sqrt((e.d[1] * e.d[1] + e.d[2] * e.d[2]) + e.d[3] * e.d[3]);
-    e.scale = std::fabs(e.d[0]) + ns;
+    e.len = ns;
     const bool finite = std::isfinite(e.d[0]) && std::isfinite(e.d[1]) &&
                         std::isfinite(e.d[2]) && std::isfinite(e.d[3]) && std::isfinite(e.dsq);
     const bool regular = finite && e.dsq >= 1e-6 * (e.d[0] * e.d[0] + ns * ns) && e.dsq > 0.0;
@@ -471,8 +471,8 @@ struct Request {
   // items: per (r, k) group the reference bubbles in summation order
   std::vector<std::vector<int32_t>> bubbles_of_real;  // selected a per realisation
   int mode = 0;  // phase-sum kernel: 0 FP64 direct, 1 FP64 blocked, 2 mixed
-  int tile_threads = 0;  // phase-sum workgroup size (0: automatic)
-  int tile_rb = 16;      // time steps per factorised phase block
+  int tile_threads = 0;  // phase-sum workgroup size (0: the form's default)
+  int tile_rb = 0;       // time steps per factorised phase block (0: the form's default)
   bool want_c = true, want_s = true;
   bool skip_uncollided = true;
   // per-item rows of tau / perm and per-item partial C (false: the fused
@@ -879,7 +879,7 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   Request qq = q;
   qq.item_buffers = false;
   qq.tile_threads = static_cast<int>(tiles.threads);
-  qq.tile_rb = tiles.block_steps ? static_cast<int>(tiles.block_steps) : 16;
+  qq.tile_rb = bgw::resolve_block_steps(static_cast<int>(tiles.block_steps), q.mode);
   Launch Lc;
   setup(E, qq, Lc);
   hipStream_t st = E.dev.stream;
@@ -1123,6 +1123,8 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   out.ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]) / std::max(1, replays);
   out.used = tiles;
   out.used.block_steps = static_cast<size_t>(qq.tile_rb);
+  out.used.threads = static_cast<size_t>(
+      bgw::resolve_threads(static_cast<int>(tiles.threads), static_cast<int>(q.n_w), q.mode));
   out.used.persistent_blocks = pipelined ? 0 : static_cast<size_t>(n_blocks);
   E.timings.geometry_ms = 0.0;
   E.timings.spectrum_ms = out.ms;
@@ -1226,7 +1228,7 @@ void check_tiles(const BgwTileConfig& t) {
 }
 
 void apply_tiles(const BgwTileConfig& t, Request& q) {
-  q.tile_rb = t.block_steps ? static_cast<int>(t.block_steps) : 16;
+  q.tile_rb = static_cast<int>(t.block_steps);  // 0: the form's default
   q.tile_threads = static_cast<int>(t.threads);
 }
 

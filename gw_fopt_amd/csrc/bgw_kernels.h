@@ -50,8 +50,8 @@ struct SpecArgs {
   double damping;
   double kfac;            // 1 / (6 pi)
   int skip_uncollided;    // drop rings whose uncollided part is rounding noise
-  int mixed_block;        // time steps per factorised phase block (8 or 16)
-  int threads;            // workgroup size (0: automatic); lanes own omegas
+  int mixed_block;        // time steps per factorised phase block (8 or 16; 0: default)
+  int threads;            // workgroup size (0: default); lanes own omegas
   int time_tables;        // per-item time tables in LDS (spectrum_time_tables)
   double* partial;        // (n_items, 2, n_sets_out, n_w) complex
   int item0;              // tau / perm / partial rows are item - item0
@@ -101,6 +101,20 @@ size_t ensemble_lds_bytes(const EnsArgs& a, int mode);
 // population standard deviation over realisations of kmean.
 hipError_t launch_ensemble_stats(const double* s_spec, int n_real, int n_k, int nq, double* kmean,
                                  double* mean, double* stdev, hipStream_t s);
+
+// Default launch shape of the phase sums (0 in SpecArgs::mixed_block /
+// threads): time steps per factorised block, 8 for the FP64 blocked form and
+// 16 otherwise; workgroups of 64..256 lanes holding one omega each in the
+// FP64 form and two in the blocked forms.
+inline int resolve_block_steps(int block_steps, int mode) {
+  return block_steps ? block_steps : (mode == 1 ? 8 : 16);
+}
+inline int resolve_threads(int threads, int n_w, int mode) {
+  if (threads > 0) return threads;
+  const int per_lane = mode == 0 ? 1 : 2;
+  int t = ((n_w + per_lane - 1) / per_lane + 63) / 64 * 64;
+  return t < 64 ? 64 : (t > 256 ? 256 : t);
+}
 
 hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s);
 hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s);

@@ -508,7 +508,7 @@ template <int NS>
 hipError_t launch_spec_ns(const SpecArgs& a, int n_items, int mode, int opl, int threads,
                           size_t lds, hipStream_t s) {
   if (mode == 0) return launch_spec_opl<NS, 0, 16>(a, n_items, opl, threads, lds, s);
-  const bool rb8 = a.mixed_block == 8;
+  const bool rb8 = resolve_block_steps(a.mixed_block, mode) == 8;
   if (mode == 1)
     return rb8 ? launch_spec_opl<NS, 1, 8>(a, n_items, opl, threads, lds, s)
                : launch_spec_opl<NS, 1, 16>(a, n_items, opl, threads, lds, s);
@@ -521,12 +521,8 @@ hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t
   if (n_items == 0) return hipSuccess;
   if (mode < 0 || mode > 2) return hipErrorInvalidValue;
   // Lanes own omegas: threads = 64..256 (a multiple of 64), up to 4 omegas
-  // per lane.  a.threads (0 = automatic) is the tunable choice.
-  int threads = a.threads;
-  if (threads <= 0) {
-    threads = ((a.n_w + 63) / 64) * 64;
-    if (threads > 256) threads = 256;
-  }
+  // per lane.  a.threads / a.mixed_block (0 = default) are the tunable choices.
+  const int threads = resolve_threads(a.threads, a.n_w, mode);
   if (threads < 64 || threads > 256 || threads % 64) return hipErrorInvalidValue;
   const int opl_needed = (a.n_w + threads - 1) / threads;
   if (opl_needed > 4) return hipErrorInvalidValue;

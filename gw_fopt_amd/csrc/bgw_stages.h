@@ -177,13 +177,19 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
         if (need) {
           const double dot = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1, x2, x3);
           if (!(dot <= 1e-10)) {
-            const double dtv = e.dsq / (2.0 * dot);
-            if (dtv != dtv) {
-              fb = true;
-            } else if (dtv > 0.0 && (dtv < best || (dtv == best && e.idx < bidx))) {
-              best = dtv;
-              bidx = e.idx;
-              bpos = s;
+            // dsq > best * 2 dot * (1 + 1e-12) proves dsq / (2 dot) > best
+            // (the relative slack exceeds the rounding of both sides), so the
+            // division is done only for candidates that can reach best.
+            const double two_dot = 2.0 * dot;
+            if (!(e.dsq > best * two_dot * (1.0 + 1e-12))) {
+              const double dtv = e.dsq / two_dot;
+              if (dtv != dtv) {
+                fb = true;
+              } else if (dtv > 0.0 && (dtv < best || (dtv == best && e.idx < bidx))) {
+                best = dtv;
+                bidx = e.idx;
+                bpos = s;
+              }
             }
           }
         }
@@ -222,7 +228,8 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       if (active) {
         double dtab = 0.0;  // compute_delta_tab (:446-468)
         if (result >= 0) {
-          const RotCand eb = fetch_cand(inv[result], cand_lds, n_lds, list, R, identity);
+          // the verified minimum was fetched already; fallback lanes look theirs up
+          const RotCand eb = fb ? fetch_cand(inv[result], cand_lds, n_lds, list, R, identity) : b0;
           const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1, x2, x3);
           dtab = fabs(dot) < 1e-10 ? 0.0 : eb.dsq / (2.0 * dot);
           if (dtab > 0.0) tau = dtab;  // AlreadyCollided needs delta_tab > 0 (:405)
@@ -321,6 +328,8 @@ __device__ __forceinline__ double cutoff_weight(const SpecArgs& g, double t) {
 }
 
 constexpr int kTimeChunk = 256;
+// Blocked forms: an exact anchor sincos every kReanchor blocks of a chunk.
+constexpr int kReanchor = 4;
 
 // Per-ring tables in LDS, laid out after each other, then the per-item time
 // tables (when they fit) and the per-chunk phase inputs.
@@ -565,10 +574,11 @@ namespace {
 // MODE 0: FP64, one double sincos per (t, omega) term (the parity form).
 // MODE 1: FP64, block-factorised phases: exp(i w psi(t0 + r)) = A(t0) E_r with
 //         E_r = exp(i w r dt (1 - cos th)); double E_r (recurrence), double
-//         anchors A, double block sums.
-// MODE 2: mixed, the same factorisation with the phase arguments reduced in
-//         double and the sincos, E_r table, weights and block sums in float;
-//         block results accumulated in float with Kahan compensation.
+//         block sums; anchors A by an exact FP64 sincos every kReanchor
+//         blocks and A(t0 + RB) = A(t0) E_RB in between.
+// MODE 2: mixed, the same factorisation and FP64 anchors, with the E_r table,
+//         weights and block sums in float; block results accumulated in float
+//         with Kahan compensation.
 //
 // spectrum_item: one item's C contribution; lane l holds omegas l + q * blockDim.x,
 // res[q][s] = (Re C+, Im C+, Re Cx, Im Cx) of set s.  tau_item indexes g.tau /
@@ -613,9 +623,11 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
       __syncthreads();
       continue;
     }
-    // E_r = exp(i omega r step (1 - cos th)), r < RB, by a double recurrence
+    // E_r = exp(i omega r step (1 - cos th)), r <= RB, by a double recurrence;
+    // E_RB advances a block anchor to the next block.
     constexpr int kE = MODE == 0 ? 1 : RB;
     Acc er[OPL][kE], ei[OPL][kE];
+    double ebr[OPL], ebi[OPL];
     if constexpr (MODE != 0) {
       const double delta = ri.t_step * (1.0 - g.ring_c[i]);
 #pragma unroll
@@ -632,6 +644,8 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           cr = nr;
           ci = ni;
         }
+        ebr[q] = cr;
+        ebi[q] = ci;
       }
     }
     for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
@@ -657,59 +671,74 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           }
         }
       } else {
-        for (int l0 = 0; l0 < len; l0 += RB) {
-          const double psi0 = L.psi[l0];
-          Acc bs[OPL][NS][4];
+        // Groups of kReanchor blocks: an exact FP64 sincos gives the anchor
+        // exp(i w psi) of the group's first block; each following block's
+        // anchor is the previous one times E_RB (FP64).  A block's weights are
+        // loaded into registers before its FMA chain so the LDS reads overlap.
+        for (int g0 = 0; g0 < len; g0 += kReanchor * RB) {
+          double anc_r[OPL], anc_i[OPL];
+          const double psi0 = L.psi[g0];
 #pragma unroll
-          for (int q = 0; q < OPL; ++q)
+          for (int q = 0; q < OPL; ++q) sincos_f64(om[q] * psi0, &anc_i[q], &anc_r[q]);
+          const int g1 = min(len, g0 + kReanchor * RB);
+          for (int l0 = g0; l0 < g1; l0 += RB) {
+            Acc vv[RB][2 * NS];  // zero-padded past len
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
+            for (int r = 0; r < RB; ++r)
 #pragma unroll
-              for (int c = 0; c < 4; ++c) bs[q][s][c] = Acc(0);
+              for (int c = 0; c < 2 * NS; ++c) {
+                if constexpr (MODE == 2) vv[r][c] = L.vf[(l0 + r) * 2 * NS + c];
+                else vv[r][c] = L.v[(l0 + r) * 2 * NS + c];
+              }
+            Acc bs[OPL][NS][4];
 #pragma unroll
-          for (int r = 0; r < RB; ++r) {
-            const Acc* v;  // zero-padded past len
-            if constexpr (MODE == 2) v = L.vf + (l0 + r) * 2 * NS;
-            else v = L.v + (l0 + r) * 2 * NS;
+            for (int q = 0; q < OPL; ++q)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-              const Acc vp = v[2 * s];
-              const Acc vx = v[2 * s + 1];
+              for (int s = 0; s < NS; ++s)
 #pragma unroll
-              for (int q = 0; q < OPL; ++q) {
-                bs[q][s][0] = fma(vp, er[q][r], bs[q][s][0]);
-                bs[q][s][1] = fma(vp, ei[q][r], bs[q][s][1]);
-                bs[q][s][2] = fma(vx, er[q][r], bs[q][s][2]);
-                bs[q][s][3] = fma(vx, ei[q][r], bs[q][s][3]);
+                for (int c = 0; c < 4; ++c) bs[q][s][c] = Acc(0);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+#pragma unroll
+              for (int s = 0; s < NS; ++s) {
+                const Acc vp = vv[r][2 * s];
+                const Acc vx = vv[r][2 * s + 1];
+#pragma unroll
+                for (int q = 0; q < OPL; ++q) {
+                  bs[q][s][0] = fma(vp, er[q][r], bs[q][s][0]);
+                  bs[q][s][1] = fma(vp, ei[q][r], bs[q][s][1]);
+                  bs[q][s][2] = fma(vx, er[q][r], bs[q][s][2]);
+                  bs[q][s][3] = fma(vx, ei[q][r], bs[q][s][3]);
+                }
               }
             }
-          }
 #pragma unroll
-          for (int q = 0; q < OPL; ++q) {
-            Acc as, ac;
-            if constexpr (MODE == 2) {
-              sincos_f64_to_f32(om[q] * psi0, &as, &ac);
-            } else {
-              sincos_f64(om[q] * psi0, &as, &ac);
-            }
+            for (int q = 0; q < OPL; ++q) {
+              const Acc as = static_cast<Acc>(anc_i[q]);
+              const Acc ac = static_cast<Acc>(anc_r[q]);
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-              // (ac + i as) * (b0 + i b1), likewise for the cross polarisation
-              const Acc p0 = fma(ac, bs[q][s][0], -as * bs[q][s][1]);
-              const Acc p1 = fma(ac, bs[q][s][1], as * bs[q][s][0]);
-              const Acc p2 = fma(ac, bs[q][s][2], -as * bs[q][s][3]);
-              const Acc p3 = fma(ac, bs[q][s][3], as * bs[q][s][2]);
-              if constexpr (MODE == 2) {
-                kahan_add(acc[q][s][0], comp[q][s][0], p0);
-                kahan_add(acc[q][s][1], comp[q][s][1], p1);
-                kahan_add(acc[q][s][2], comp[q][s][2], p2);
-                kahan_add(acc[q][s][3], comp[q][s][3], p3);
-              } else {
-                acc[q][s][0] += p0;
-                acc[q][s][1] += p1;
-                acc[q][s][2] += p2;
-                acc[q][s][3] += p3;
+              for (int s = 0; s < NS; ++s) {
+                // (ac + i as) * (b0 + i b1), likewise for the cross polarisation
+                const Acc p0 = fma(ac, bs[q][s][0], -as * bs[q][s][1]);
+                const Acc p1 = fma(ac, bs[q][s][1], as * bs[q][s][0]);
+                const Acc p2 = fma(ac, bs[q][s][2], -as * bs[q][s][3]);
+                const Acc p3 = fma(ac, bs[q][s][3], as * bs[q][s][2]);
+                if constexpr (MODE == 2) {
+                  kahan_add(acc[q][s][0], comp[q][s][0], p0);
+                  kahan_add(acc[q][s][1], comp[q][s][1], p1);
+                  kahan_add(acc[q][s][2], comp[q][s][2], p2);
+                  kahan_add(acc[q][s][3], comp[q][s][3], p3);
+                } else {
+                  acc[q][s][0] += p0;
+                  acc[q][s][1] += p1;
+                  acc[q][s][2] += p2;
+                  acc[q][s][3] += p3;
+                }
               }
+              const double nr = anc_r[q] * ebr[q] - anc_i[q] * ebi[q];
+              const double ni = anc_r[q] * ebi[q] + anc_i[q] * ebr[q];
+              anc_r[q] = nr;
+              anc_i[q] = ni;
             }
           }
         }

@@ -226,3 +226,19 @@ def test_c_integrand_async_selected():
     late = int(np.argmax(inter[:, 0]))
     Z = bf.compute_c_integrand(w, t_end=inter[late, 0] * 0.5, n_t=10, selected_bubbles=[late])
     assert np.all(Z == 0)
+
+
+@pytest.mark.parametrize("seed", [17, 29])
+def test_masks_bitexact_dense_rotated(seed):
+    """40 staggered bubbles in a periodic box (280 candidates per bubble),
+    24 x 48 directions, three k-hat frames: every first collider and delay
+    equals the oracle's (the culled scans must decide exactly as the full scan)."""
+    b = nucleation.exponential_nucleation(18.0, n_max=40, seed=seed)
+    ext = nucleation.face_images(b, 1.0)
+    bf = flow(b, ext, 24, 48, [[1.0]], [[5.0]])
+    for k in mb.fibonacci_directions(3):
+        for a in range(0, len(b), 3):
+            f, d = bf.first_colliding_in_frame(a, k)
+            fo, do = oracle.first_colliding(b, ext, 24, 48, a, k_hat=k)
+            np.testing.assert_array_equal(f, fo)
+            assert d.tobytes() == do.tobytes()
