@@ -329,7 +329,7 @@ __device__ __forceinline__ double cutoff_weight(const SpecArgs& g, double t) {
 
 constexpr int kTimeChunk = 256;
 // Blocked forms: an exact anchor sincos every kReanchor blocks of a chunk.
-constexpr int kReanchor = 4;
+constexpr int kReanchor = 16;
 
 // Per-ring tables in LDS, laid out after each other, then the per-item time
 // tables (when they fit) and the per-chunk phase inputs.
