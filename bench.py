@@ -119,6 +119,17 @@ def pmc_copy_calibration():
         return None
 
 
+def pmc_flops(mode: str, n_k: int):
+    """Executed FP64 VALU flops per phase-sum launch: SQ_INSTS_VALU_FLOPS_FP64
+    counts flops per wave instruction (an FMA counts 2), times 64 lanes."""
+    try:
+        prog = ["--profile-mode", mode, "--steps", "2", "--warmup", "1", "--n-k", str(n_k)]
+        v = _mean_for(_pmc_run("SQ_INSTS_VALU_FLOPS_FP64", prog), "spectrum_kernel")
+        return None if v is None else 64.0 * v
+    except Exception:
+        return None
+
+
 def pmc_traffic(mode: str, n_k: int, calibration):
     """HBM bytes per phase-sum launch: FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch) of the profiled kernel, each scaled by bytes per unit measured the
@@ -286,11 +297,12 @@ def main():
 
     # CPU-side inputs and counter runs before this process touches the GPU
     ens = None if args.no_ensemble else configs.ensemble_512()
-    traffic = {}
+    traffic, executed = {}, {}
     if not args.no_pmc:
         cal = pmc_copy_calibration()
         for m in ("fp64_blocked", "mixed"):
             traffic[m] = pmc_traffic(m, args.n_k, cal)
+        executed["fp64_blocked"] = pmc_flops("fp64_blocked", args.n_k)
 
     from gw_fopt_amd import many_bubbles as mb
     lb = mb.LatticeBubbles(mb.EmptyLattice(), wl.interior, wl.exterior)
@@ -346,7 +358,13 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": None if tr is None else tr["bytes"],
                      "flops_per_launch": spec_flops, "launch_ms": r["spectrum_ms"],
-                     "hbm_GBs": None if tr is None else tr["bytes"] / (r["spectrum_ms"] * 1e-3) / 1e9},
+                     "hbm_GBs": None if tr is None else tr["bytes"] / (r["spectrum_ms"] * 1e-3) / 1e9,
+                     "executed_fp64_flops_per_launch": executed.get(main_mode),
+                     "executed_tflops": (None if executed.get(main_mode) is None else
+                                         executed[main_mode] / (r["spectrum_ms"] * 1e-3) / 1e12),
+                     "executed_frac": (None if executed.get(main_mode) is None else
+                                       executed[main_mode] / (r["spectrum_ms"] * 1e-3) / 1e12
+                                       / FP64_PEAK_TFLOPS)},
         "modes": {m: {"points_per_s": points / (v["device_ms"] * 1e-3),
                       "wall_points_per_s": points / (v["wall_ms"] * 1e-3),
                       "err_vs_fp64": errs.get(m), **v}
