@@ -18,6 +18,7 @@ namespace {
 constexpr int kGeoThreads = 256;
 constexpr int kGeoLdsCand = 128;
 constexpr int kSortCap = 1024;
+constexpr int kDirPerLane = 1;  // surface directions per lane in the first-collider scan
 constexpr double kInf = __builtin_inf();
 
 // A candidate in the k-hat frame, as staged in LDS.
@@ -109,6 +110,137 @@ __device__ __forceinline__ bool wave_any(bool p) { return __any(p ? 1 : 0) != 0;
 
 namespace {
 
+// Bitonic sort of the P slots of one ring held by one wavefront: lane l
+// holds ring-local elements l + 64 r (r < P / 64) when P >= 64, or element
+// l % P of ring l / P when P < 64.  Partners less than 64 apart are exchanged
+// across lanes by shuffles, farther ones inside a lane; no LDS traffic and no
+// workgroup barrier.  Keys are (delay, phi index) pairs, all distinct.
+template <int P>
+__device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, int gcount, int i0,
+                             int n_phi, double* tau_out, uint16_t* perm_out) {
+  constexpr int EPL = P >= 64 ? P / 64 : 1;   // elements per lane
+  constexpr int RPW = P >= 64 ? 1 : 64 / P;   // rings per wavefront
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int n_waves = blockDim.x >> 6;
+  const int e0 = P >= 64 ? lane : (lane % P);
+  const int rsub = P >= 64 ? 0 : lane / P;
+  for (int rb = wave * RPW; rb < gcount; rb += n_waves * RPW) {
+    const int rr = rb + rsub;
+    const bool valid = rr < gcount;
+    double key[EPL];
+    int32_t val[EPL];
+#pragma unroll
+    for (int r = 0; r < EPL; ++r) {
+      const int e = e0 + 64 * r;
+      key[r] = valid ? sort_key[rr * P + e] : kInf;
+      val[r] = valid ? sort_val[rr * P + e] : 0x7fffffff - e;
+    }
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        if (j >= 64) {
+#pragma unroll
+          for (int r = 0; r < EPL; ++r) {
+            const int rp = r ^ (j >> 6);
+            if (rp > r) {
+              const int e = e0 + 64 * r;
+              const bool up = (e & k) == 0;
+              const bool greater = key[r] > key[rp] || (key[r] == key[rp] && val[r] > val[rp]);
+              if (greater == up) {
+                const double tk = key[r];
+                key[r] = key[rp];
+                key[rp] = tk;
+                const int32_t tv = val[r];
+                val[r] = val[rp];
+                val[rp] = tv;
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < EPL; ++r) {
+            const int e = e0 + 64 * r;
+            const double pk = __shfl_xor(key[r], j, 64);
+            const int32_t pv = __shfl_xor(val[r], j, 64);
+            const bool lower = (e & j) == 0;
+            const bool up = (e & k) == 0;
+            const bool mine_greater = key[r] > pk || (key[r] == pk && val[r] > pv);
+            // the lower element of the pair keeps the minimum when ascending
+            const bool take = (lower == up) ? mine_greater : !mine_greater;
+            if (take) {
+              key[r] = pk;
+              val[r] = pv;
+            }
+          }
+        }
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int r = 0; r < EPL; ++r) {
+        const int e = e0 + 64 * r;
+        if (e < n_phi) {
+          tau_out[static_cast<int64_t>(i0 + rr) * n_phi + e] = key[r];
+          perm_out[static_cast<int64_t>(i0 + rr) * n_phi + e] = static_cast<uint16_t>(val[r]);
+        }
+      }
+    }
+  }
+}
+
+// Rings of more than 128 slots: the same network over the workgroup in LDS,
+// one barrier per stage.
+__device__ void sort_rings_lds(int P, double* sort_key, int32_t* sort_val, int gcount, int i0,
+                               int n_phi, double* tau_out, uint16_t* perm_out) {
+  const int nslot = gcount * P;
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
+        const int p = e ^ jj;
+        if (p > e) {
+          // direction from the index within the ring's segment, so every
+          // segment ends ascending (k == P merges the whole segment)
+          const bool up = ((e & (P - 1)) & k) == 0;
+          const double ke = sort_key[e], kp = sort_key[p];
+          const bool greater = ke > kp || (ke == kp && sort_val[e] > sort_val[p]);
+          if (greater == up) {
+            sort_key[e] = kp;
+            sort_key[p] = ke;
+            const int32_t t = sort_val[e];
+            sort_val[e] = sort_val[p];
+            sort_val[p] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
+    const int ring = i0 + e / P;
+    const int r = e % P;
+    if (r < n_phi) {
+      tau_out[static_cast<int64_t>(ring) * n_phi + r] = sort_key[e];
+      perm_out[static_cast<int64_t>(ring) * n_phi + r] = static_cast<uint16_t>(sort_val[e]);
+    }
+  }
+}
+
+__device__ void sort_rings(int P, double* sort_key, int32_t* sort_val, int gcount, int i0,
+                           int n_phi, double* tau_out, uint16_t* perm_out) {
+  switch (P) {
+    case 2: sort_rings_p<2>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    case 4: sort_rings_p<4>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    case 8: sort_rings_p<8>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    case 16: sort_rings_p<16>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    case 32: sort_rings_p<32>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    case 64: sort_rings_p<64>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    case 128: sort_rings_p<128>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+    default: sort_rings_lds(P, sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
+  }
+}
+
 // One item's first colliders and sorted per-ring delays, written to
 // tau_out / perm_out (n_c * n_phi).  Called by the whole workgroup; the
 // LDS arrays hold kGeoLdsCand candidates and kSortCap sort slots.  Ends
@@ -135,8 +267,11 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
   const int G = kSortCap / P;  // rings per sort group
   const int64_t n_dir = static_cast<int64_t>(g.n_c) * n_phi;
 
-  // Each wavefront takes a compact patch of ph rings x pw azimuths, so its
-  // lanes share first colliders and stop their culled scans together.
+  // Each wavefront takes kDirPerLane neighbouring patches of ph rings x pw
+  // azimuths, one direction of each per lane: its lanes share first colliders
+  // and stop their culled scans together, and every candidate fetched and
+  // every loop step serves kDirPerLane directions.
+  constexpr int D = kDirPerLane;
   const int pw = P < 8 ? P : 8;
   const int ph = 64 / pw;
   const int patches_per_row = P / pw;
@@ -144,143 +279,161 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
     const int gcount = min(G, g.n_c - i0);
     const int nslot = gcount * P;
     const int n_patch = ((gcount + ph - 1) / ph) * patches_per_row;
-    const int nwork_round = ((n_patch * 64 + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    const int n_multi = (n_patch + D - 1) / D;
+    const int nwork_round = ((n_multi * 64 + blockDim.x - 1) / blockDim.x) * blockDim.x;
     for (int w = threadIdx.x; w < nwork_round; w += blockDim.x) {
-      const int patch = w >> 6, lane = w & 63;
-      const int prow = patch / patches_per_row, pcol = patch % patches_per_row;
-      const int rring = prow * ph + lane / pw;  // ring within the group
-      const int j = pcol * pw + lane % pw;
-      const int ring = i0 + rring;
-      const int slot = rring * P + j;           // sort-buffer position
-      const bool in_group = patch < n_patch && rring < gcount;
-      const bool active = in_group && j < n_phi;
-      double x1 = 0.0, x2 = 0.0, x3 = 0.0;
-      if (active) {
-        const double st = g.sin_t[ring];
-        x1 = st * g.cos_p[j];
-        x2 = st * g.sin_p[j];
-        x3 = g.cos_t[ring];
+      const int lane = w & 63;
+      int ring[D], jd[D], slot[D];
+      bool in_group[D], active[D], fb[D], act[D];
+      double x1[D], x2[D], x3[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int patch = (w >> 6) * D + d;
+        const int prow = patch / patches_per_row, pcol = patch % patches_per_row;
+        const int rring = prow * ph + lane / pw;  // ring within the group
+        jd[d] = pcol * pw + lane % pw;
+        ring[d] = i0 + rring;
+        slot[d] = rring * P + jd[d];              // sort-buffer position
+        in_group[d] = patch < n_patch && rring < gcount;
+        active[d] = in_group[d] && jd[d] < n_phi;
+        x1[d] = 0.0;
+        x2[d] = 0.0;
+        x3[d] = 0.0;
+        if (active[d]) {
+          const double st = g.sin_t[ring[d]];
+          x1[d] = st * g.cos_p[jd[d]];
+          x2[d] = st * g.sin_p[jd[d]];
+          x3[d] = g.cos_t[ring[d]];
+        }
+        fb[d] = active[d] &&
+                (ri.nonfinite || !(isfinite(x1[d]) && isfinite(x2[d]) && isfinite(x3[d])));
+        act[d] = active[d] && !fb[d];
       }
-      bool fb = active && (ri.nonfinite || !(isfinite(x1) && isfinite(x2) && isfinite(x3)));
-      const bool act = active && !fb;
 
       // Pass 1: minimum collision delay over the candidates (dot > 1e-10,
       // delay > 0), ties to the lowest bubble index.  The list is sorted by a
       // lower bound `key` of the delay: once key * (1 - 1e-7) > best for every
-      // lane, no later candidate can reach or tie a lane's best.
-      double best = kInf;
-      int bidx = 0x7fffffff, bpos = -1;
+      // direction of the wave, no later candidate can reach or tie a best.
+      // dsq > best * 2 dot * (1 + 1e-12) proves dsq / (2 dot) > best (the
+      // relative slack exceeds the rounding of both sides), so the division
+      // runs only when some direction has a candidate that can reach its best.
+      double best[D], two_dot[D];
+      int bidx[D], bpos[D];
+      bool cand[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        best[d] = kInf;
+        bidx[d] = 0x7fffffff;
+        bpos[d] = -1;
+      }
       for (int s = 0; s < n_cand; ++s) {
         const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
-        const bool need = act && !(e.key * (1.0 - 1e-7) > best);
-        if (!wave_any(need)) break;
-        if (need) {
-          const double dot = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1, x2, x3);
-          if (!(dot <= 1e-10)) {
-            // dsq > best * 2 dot * (1 + 1e-12) proves dsq / (2 dot) > best
-            // (the relative slack exceeds the rounding of both sides), so the
-            // division is done only for candidates that can reach best.
-            const double two_dot = 2.0 * dot;
-            if (!(e.dsq > best * two_dot * (1.0 + 1e-12))) {
-              const double dtv = e.dsq / two_dot;
+        bool any_need = false, any_cand = false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const bool need = act[d] && !(e.key * (1.0 - 1e-7) > best[d]);
+          any_need = any_need || need;
+          const double dot = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1[d], x2[d], x3[d]);
+          two_dot[d] = 2.0 * dot;
+          cand[d] = need && !(dot <= 1e-10) && !(e.dsq > best[d] * two_dot[d] * (1.0 + 1e-12));
+          any_cand = any_cand || cand[d];
+        }
+        if (!wave_any(any_need)) break;
+        if (wave_any(any_cand)) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            if (cand[d]) {
+              const double dtv = e.dsq / two_dot[d];
               if (dtv != dtv) {
-                fb = true;
-              } else if (dtv > 0.0 && (dtv < best || (dtv == best && e.idx < bidx))) {
-                best = dtv;
-                bidx = e.idx;
-                bpos = s;
+                fb[d] = true;
+              } else if (dtv > 0.0 && (dtv < best[d] || (dtv == best[d] && e.idx < bidx[d]))) {
+                best[d] = dtv;
+                bidx[d] = e.idx;
+                bpos[d] = s;
               }
             }
           }
         }
       }
 
-      // Pass 2: verify the minimum against every other bubble c with the
+      // Pass 2: verify each minimum against every other bubble c with the
       // reference's cross-multiplied test.  For a regular, non-grazing
       // candidate, bubbles with key > best * (1 + 1e-4) provably pass, so the
       // wave again walks a prefix of the list.
-      const bool p2 = act && !fb && bpos >= 0;
-      RotCand b0{};
-      double thr = kInf;
-      if (p2) {
-        b0 = fetch_cand(bpos, cand_lds, n_lds, list, R, identity);
-        const double scale = fabs(b0.d0) + sqrt(b0.d1 * b0.d1 + b0.d2 * b0.d2 + b0.d3 * b0.d3);
-        const bool regular = (b0.flags & kCandRegular) != 0;
-        const bool grazing = !(best <= 1e3 * scale);
-        if (regular && !grazing) thr = best * (1.0 + 1e-4);
+      bool p2[D], ok[D];
+      RotCand b0[D];
+      double thr[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        p2[d] = act[d] && !fb[d] && bpos[d] >= 0;
+        ok[d] = true;
+        thr[d] = kInf;
+        b0[d] = RotCand{};
+        if (p2[d]) {
+          b0[d] = fetch_cand(bpos[d], cand_lds, n_lds, list, R, identity);
+          const double scale =
+              fabs(b0[d].d0) + sqrt(b0[d].d1 * b0[d].d1 + b0[d].d2 * b0[d].d2 + b0[d].d3 * b0[d].d3);
+          const bool regular = (b0[d].flags & kCandRegular) != 0;
+          const bool grazing = !(best[d] <= 1e3 * scale);
+          if (regular && !grazing) thr[d] = best[d] * (1.0 + 1e-4);
+        }
       }
-      bool ok = true;
       for (int s = 0; s < n_cand; ++s) {
         const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
-        const bool need = p2 && ok && !(e.key > thr);
-        if (!wave_any(need)) break;
-        if (need && s != bpos && !collision_point_ok(b0, e, x1, x2, x3)) ok = false;
+        bool need[D], any_need = false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          need[d] = p2[d] && ok[d] && !(e.key > thr[d]);
+          any_need = any_need || need[d];
+        }
+        if (!wave_any(any_need)) break;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          if (need[d] && s != bpos[d] && !collision_point_ok(b0[d], e, x1[d], x2[d], x3[d]))
+            ok[d] = false;
       }
-      if (p2 && !ok) fb = true;
 
-      int result = -1;
-      if (fb) {
-        result = reference_scan(list, inv, it.a, ri.n_int, ri.n_tot, R, identity, x1, x2, x3);
-      } else if (p2) {
-        result = bidx;
-      }
-      double tau = kInf;
-      if (active) {
-        double dtab = 0.0;  // compute_delta_tab (:446-468)
-        if (result >= 0) {
-          // the verified minimum was fetched already; fallback lanes look theirs up
-          const RotCand eb = fb ? fetch_cand(inv[result], cand_lds, n_lds, list, R, identity) : b0;
-          const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1, x2, x3);
-          dtab = fabs(dot) < 1e-10 ? 0.0 : eb.dsq / (2.0 * dot);
-          if (dtab > 0.0) tau = dtab;  // AlreadyCollided needs delta_tab > 0 (:405)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (p2[d] && !ok[d]) fb[d] = true;
+        int result = -1;
+        if (fb[d]) {
+          result = reference_scan(list, inv, it.a, ri.n_int, ri.n_tot, R, identity, x1[d], x2[d],
+                                  x3[d]);
+        } else if (p2[d]) {
+          result = bidx[d];
         }
-        const int64_t dir = static_cast<int64_t>(ring) * n_phi + j;
-        if (g.first_out) {
-          int code = -1;
-          if (result >= 0) code = result < ri.n_int ? result : -(result - ri.n_int) - 2;
-          g.first_out[static_cast<int64_t>(item_id) * n_dir + dir] = code;
+        double tau = kInf;
+        if (active[d]) {
+          double dtab = 0.0;  // compute_delta_tab (:446-468)
+          if (result >= 0) {
+            // the verified minimum was fetched already; fallback lanes look theirs up
+            const RotCand eb =
+                fb[d] ? fetch_cand(inv[result], cand_lds, n_lds, list, R, identity) : b0[d];
+            const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1[d], x2[d], x3[d]);
+            dtab = fabs(dot) < 1e-10 ? 0.0 : eb.dsq / (2.0 * dot);
+            if (dtab > 0.0) tau = dtab;  // AlreadyCollided needs delta_tab > 0 (:405)
+          }
+          const int64_t dir = static_cast<int64_t>(ring[d]) * n_phi + jd[d];
+          if (g.first_out) {
+            int code = -1;
+            if (result >= 0) code = result < ri.n_int ? result : -(result - ri.n_int) - 2;
+            g.first_out[static_cast<int64_t>(item_id) * n_dir + dir] = code;
+          }
+          if (g.dtab_out) g.dtab_out[static_cast<int64_t>(item_id) * n_dir + dir] = dtab;
         }
-        if (g.dtab_out) g.dtab_out[static_cast<int64_t>(item_id) * n_dir + dir] = dtab;
-      }
-      if (in_group) {
-        sort_key[slot] = tau;
-        sort_val[slot] = j;
+        if (in_group[d]) {
+          sort_key[slot[d]] = tau;
+          sort_val[slot[d]] = jd[d];
+        }
       }
     }
     __syncthreads();
-    // Bitonic sort of every P-slot segment (one ring each), ascending in
-    // (delay, phi index).  Padding slots carry phi index >= n_phi, so they
-    // sort after the directions that never collide (delay +inf as well).
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int jj = k >> 1; jj > 0; jj >>= 1) {
-        for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
-          const int p = e ^ jj;
-          if (p > e) {
-            // direction from the index within the ring's segment, so every
-            // segment ends ascending (k == P merges the whole segment)
-            const bool up = ((e & (P - 1)) & k) == 0;
-            const double ke = sort_key[e], kp = sort_key[p];
-            const bool greater = ke > kp || (ke == kp && sort_val[e] > sort_val[p]);
-            if (greater == up) {
-              sort_key[e] = kp;
-              sort_key[p] = ke;
-              const int32_t t = sort_val[e];
-              sort_val[e] = sort_val[p];
-              sort_val[p] = t;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
-      const int ring = i0 + e / P;
-      const int r = e % P;
-      if (r < n_phi) {
-        tau_out[static_cast<int64_t>(ring) * n_phi + r] = sort_key[e];
-        perm_out[static_cast<int64_t>(ring) * n_phi + r] = static_cast<uint16_t>(sort_val[e]);
-      }
-    }
+    // Sort every ring's P slots ascending in (delay, phi index), one ring per
+    // wavefront in registers (P <= 128).  Padding slots carry phi index >=
+    // n_phi, so they sort after the directions that never collide (delay +inf
+    // as well).
+    sort_rings(P, sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out);
     __syncthreads();
   }
 }
