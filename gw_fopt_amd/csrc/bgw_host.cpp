@@ -1318,6 +1318,41 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
 
 }  // namespace
 
+namespace {
+
+// Request over every realisation of an ensemble (all bubbles, shared k-hat list).
+Request ensemble_request(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
+                         const double* k_hat, size_t n_k, const double* t_begin,
+                         const double* t_end, size_t n_t, BgwPrecision precision) {
+  const size_t n_real = ensemble->reals.size();
+  slice_from_raw(w_arr, checked_product(n_real, n_w, "w_arr"), "w_arr");
+  slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
+  slice_from_raw(t_begin, n_real, "t_begin");
+  slice_from_raw(t_end, n_real, "t_end");
+  if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+  check_unit_vectors(k_hat, n_k);
+  Request q;
+  q.res = &need_resolution(ensemble->res);
+  q.sc = &ensemble->sc;
+  q.n_w = n_w;
+  q.n_t = n_t;
+  q.identity = false;
+  for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
+  q.omega.assign(w_arr, w_arr + n_real * n_w);
+  q.t_begin.assign(t_begin, t_begin + n_real);
+  q.t_end.assign(t_end, t_end + n_real);
+  for (size_t r = 0; r < n_real; ++r) {
+    check_time_range(q.t_begin[r], q.t_end[r]);
+    std::vector<int32_t> ids(ensemble->reals[r].n_int);
+    std::iota(ids.begin(), ids.end(), 0);
+    q.bubbles_of_real.push_back(ids);
+  }
+  q.mode = kernel_mode(precision);
+  return q;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* bgw_version(void) { return "0.1.0"; }
@@ -1768,41 +1803,6 @@ BgwStatus bgw_ensemble_set_gradient_scaling_params(bgw_ensemble* ensemble,
     ensemble->sc = make_scaling(coefficients, powers, n_sets, n_terms, damping_width);
   });
 }
-
-namespace {
-
-// Request over every realisation of an ensemble (all bubbles, shared k-hat list).
-Request ensemble_request(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
-                         const double* k_hat, size_t n_k, const double* t_begin,
-                         const double* t_end, size_t n_t, BgwPrecision precision) {
-  const size_t n_real = ensemble->reals.size();
-  slice_from_raw(w_arr, checked_product(n_real, n_w, "w_arr"), "w_arr");
-  slice_from_raw(k_hat, checked_product(n_k, 3, "k_hat"), "k_hat");
-  slice_from_raw(t_begin, n_real, "t_begin");
-  slice_from_raw(t_end, n_real, "t_end");
-  if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
-  check_unit_vectors(k_hat, n_k);
-  Request q;
-  q.res = &need_resolution(ensemble->res);
-  q.sc = &ensemble->sc;
-  q.n_w = n_w;
-  q.n_t = n_t;
-  q.identity = false;
-  for (size_t k = 0; k < n_k; ++k) q.k.push_back({k_hat[3 * k], k_hat[3 * k + 1], k_hat[3 * k + 2]});
-  q.omega.assign(w_arr, w_arr + n_real * n_w);
-  q.t_begin.assign(t_begin, t_begin + n_real);
-  q.t_end.assign(t_end, t_end + n_real);
-  for (size_t r = 0; r < n_real; ++r) {
-    check_time_range(q.t_begin[r], q.t_end[r]);
-    std::vector<int32_t> ids(ensemble->reals[r].n_int);
-    std::iota(ids.begin(), ids.end(), 0);
-    q.bubbles_of_real.push_back(ids);
-  }
-  q.mode = kernel_mode(precision);
-  return q;
-}
-
-}  // namespace
 
 BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_arr, size_t n_w,
                                         const double* k_hat, size_t n_k, const double* t_begin,

@@ -375,8 +375,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
 #pragma unroll
               for (int c = 0; c < 2 * NS; ++c)
                 w[r][c] = __builtin_fma(gr[c], zr, __builtin_fma(gi[c], zi, w[r][c]));
-              const double nr = zr * er - zi * ei;
-              const double ni = zr * ei + zi * er;
+              const double nr = __builtin_fma(zr, er, -(zi * ei));
+              const double ni = __builtin_fma(zr, ei, zi * er);
               zr = nr;
               zi = ni;
             }
