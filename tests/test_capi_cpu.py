@@ -156,3 +156,113 @@ def test_no_device_fails_loudly():
     bf.set_resolution(4, 5)
     with pytest.raises(ValueError, match="no HIP device"):
         bf.compute_c_integral([0.5, 1.0], t_begin=0.0, t_end=8.0, n_t=40)
+
+
+def _handle(inter, ext=None):
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    ext = np.zeros((0, 4)) if ext is None else ext
+    assert L.bgw_generalized_bulk_flow_empty_lattice_new(ctypes.byref(h), _lib.dptr(inter), len(inter),
+                                                         _lib.dptr(ext), len(ext)) == _lib.OK
+    return h
+
+
+def test_integrand_validation_and_zero_cases():
+    """compute_c_integrand (generalized_bulk_flow.rs:875-920): n_t is checked
+    first, an empty selection or bubbles nucleated at or after t_end give zeros
+    without touching a device, and a wrong out_len is rejected."""
+    L = _lib.lib()
+    inter = np.array([[0.0, 0.0, 10.0, 0.0], [2.0, 0.0, 0.0, 0.0]])
+    h = _handle(inter)
+    L.bgw_generalized_bulk_flow_set_resolution(h, 4, 5, True)
+    w = np.array([0.5, 1.0])
+    out = np.full(2 * 1 * 2 * 6 * 2, 7.0)
+    st = L.bgw_generalized_bulk_flow_compute_c_integrand(h, _lib.dptr(w), 2, True, 0.0, 1.0, 1, None,
+                                                         0, _lib.dptr(out), 24)
+    assert st == _lib.CALCULATION_ERROR and _lib.last_error() == "Invalid resolution: n_t must be >= 2"
+    sel = np.array([1], dtype=np.uintp)  # nucleated at t = 2 > t_end = 1: zeros, no device
+    st = L.bgw_generalized_bulk_flow_compute_c_integrand(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6,
+                                                         _lib.sptr(sel), 1, _lib.dptr(out), 24)
+    assert st == _lib.OK and np.all(out == 0.0), _lib.last_error()
+    # ... even with t_begin > t_end: that bubble never reaches the range check (:681-692)
+    st = L.bgw_generalized_bulk_flow_compute_c_integrand(h, _lib.dptr(w), 2, True, 1.5, 1.0, 6,
+                                                         _lib.sptr(sel), 1, _lib.dptr(out), 24)
+    assert st == _lib.OK
+    st = L.bgw_generalized_bulk_flow_compute_c_integrand(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6,
+                                                         _lib.sptr(sel), 1, _lib.dptr(out), 23)
+    assert "out_len=23 does not match expected length 24" in _lib.last_error()
+    L.bgw_generalized_bulk_flow_free(h)
+
+
+def test_spectrum_and_vjp_validation():
+    L = _lib.lib()
+    h = _handle(np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0]]))
+    L.bgw_generalized_bulk_flow_set_resolution(h, 4, 5, True)
+    w = np.array([0.5, 1.0])
+    k = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]])
+    cut = _lib.cutoff_config(None)
+    S = np.zeros(4)
+    st = L.bgw_generalized_bulk_flow_compute_spectrum(h, _lib.dptr(w), 2, _lib.dptr(k), 2, False, 0.0,
+                                                      1.0, 10, cut, _lib.FP64, _lib.dptr(S), 3, None,
+                                                      0, None, 0)
+    assert st == _lib.CALCULATION_ERROR and "spectrum_len=3" in _lib.last_error()
+    bad = np.array([[0.0, 0.0, 2.0]])
+    st = L.bgw_generalized_bulk_flow_compute_spectrum(h, _lib.dptr(w), 2, _lib.dptr(bad), 1, False, 0.0,
+                                                      1.0, 10, cut, _lib.FP64, _lib.dptr(S), 2, None,
+                                                      0, None, 0)
+    assert _lib.last_error() == "k_hat[0] is not a unit vector"
+    g = np.zeros(2 * 2 * 2 * 2)
+    dc = np.zeros(1)
+    dp = np.zeros(1)
+    st = L.bgw_generalized_bulk_flow_spectrum_vjp(h, _lib.dptr(w), 2, _lib.dptr(k), 2, False, 0.0, 1.0,
+                                                  10, cut, _lib.dptr(g), 7, _lib.dptr(dc), _lib.dptr(dp), 1)
+    assert "grad_c_len=7 does not match expected length 8" in _lib.last_error()
+    st = L.bgw_generalized_bulk_flow_spectrum_vjp(h, _lib.dptr(w), 2, _lib.dptr(k), 2, False, 0.0, 1.0,
+                                                  10, cut, _lib.dptr(g), 8, _lib.dptr(dc), _lib.dptr(dp), 2)
+    assert "grad_len=2 does not match expected length 1" in _lib.last_error()
+    st = L.bgw_generalized_bulk_flow_spectrum_vjp(h, _lib.dptr(w), 2, _lib.dptr(k), 2, False, 0.0, 1.0,
+                                                  1, cut, _lib.dptr(g), 8, _lib.dptr(dc), _lib.dptr(dp), 1)
+    assert _lib.last_error() == "Invalid resolution: n_t must be >= 2 for integration"
+    t = _lib.BgwTileConfig(5, 0, 0, 0)
+    assert L.bgw_generalized_bulk_flow_set_tiles(h, ctypes.byref(t)) == _lib.CALCULATION_ERROR
+    assert _lib.last_error() == "tiles.block_steps must be 0, 8 or 16"
+    t = _lib.BgwTileConfig(8, 96, 0, 0)
+    assert L.bgw_generalized_bulk_flow_set_tiles(h, ctypes.byref(t)) == _lib.CALCULATION_ERROR
+    L.bgw_generalized_bulk_flow_free(h)
+
+
+def test_ensemble_validation():
+    L = _lib.lib()
+    inter = np.array([[0.0, 0.1, 0.1, 0.1], [0.0, 0.6, 0.6, 0.6], [0.0, 0.3, 0.3, 0.3]])
+    io = np.array([0, 2, 3], dtype=np.uintp)
+    eo = np.array([0, 0, 0], dtype=np.uintp)
+    h = ctypes.c_void_p()
+    assert L.bgw_ensemble_new(None, _lib.dptr(inter), _lib.sptr(io), None, _lib.sptr(eo), 2) == _lib.NULL_POINTER
+    bad = np.array([0, 2, 1], dtype=np.uintp)
+    st = L.bgw_ensemble_new(ctypes.byref(h), _lib.dptr(inter), _lib.sptr(bad), None, _lib.sptr(eo), 2)
+    assert st == _lib.CALCULATION_ERROR and _lib.last_error() == "offsets must be non-decreasing"
+    st = L.bgw_ensemble_new(ctypes.byref(h), _lib.dptr(inter), _lib.sptr(io), None, _lib.sptr(eo), 2)
+    assert st == _lib.OK
+    assert L.bgw_ensemble_set_resolution(h, 1, 8) == _lib.CALCULATION_ERROR
+    assert L.bgw_ensemble_set_resolution(h, 4, 8) == _lib.OK
+    w = np.array([[0.5, 1.0], [0.5, 1.0]])
+    k = np.array([[0.0, 0.0, 1.0]])
+    te = np.array([1.0, 1.0])
+    S = np.zeros(4)
+    st = L.bgw_ensemble_compute_spectrum(h, _lib.dptr(w), 2, _lib.dptr(k), 1, None, _lib.dptr(te), 10,
+                                         _lib.MIXED, None, _lib.dptr(S), 4, None, 0, None, None, 0)
+    assert st == _lib.CALCULATION_ERROR and _lib.last_error() == "t_begin must not be null"
+    tb = np.zeros(2)
+    st = L.bgw_ensemble_compute_spectrum(h, _lib.dptr(w), 2, _lib.dptr(k), 1, _lib.dptr(tb), _lib.dptr(te),
+                                         10, _lib.MIXED, None, _lib.dptr(S), 3, None, 0, None, None, 0)
+    assert "spectrum_len=3 does not match expected length 4" in _lib.last_error()
+    st = L.bgw_ensemble_compute_spectrum(h, _lib.dptr(w), 2, _lib.dptr(k), 1, _lib.dptr(tb), _lib.dptr(te),
+                                         10, 9, None, _lib.dptr(S), 4, None, 0, None, None, 0)
+    assert st == _lib.INVALID_ARGUMENT
+    t = _lib.BgwTileConfig(0, 0, 0, 70000)
+    st = L.bgw_ensemble_compute_spectrum(h, _lib.dptr(w), 2, _lib.dptr(k), 1, _lib.dptr(tb), _lib.dptr(te),
+                                         10, _lib.MIXED, ctypes.byref(t), _lib.dptr(S), 4, None, 0, None,
+                                         None, 0)
+    assert _lib.last_error() == "tiles.persistent_blocks must be <= 65536"
+    L.bgw_ensemble_free(h)
+    L.bgw_ensemble_free(None)
