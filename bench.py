@@ -262,6 +262,7 @@ def ensemble_bench(ens, steps):
             "ensemble_mean_max": float(np.max(np.abs(res[m]["mean"] - ref["mean"]) / np.abs(ref["mean"])))}
         out[m]["speedup_vs_fp64"] = out[m]["points_per_s"] / out["fp64"]["points_per_s"]
     counts = np.array([len(b) for b in ens.interiors])
+    del E  # release the batch's device buffers before the rest of the run
     return {"workload": "ensemble_512: 512 realisations, beta L ~ U[10,30], bubble counts "
                         f"{int(counts.min())}-{int(counts.max())} (mean {counts.mean():.1f}), "
                         "32x64, 500 t to each percolation time, 128 omegas geomspace(0.5,300)/L, "
@@ -379,17 +380,17 @@ def main():
                         "achieved_tflops": flops["geometry"] / (r["geometry_ms"] * 1e-3) / 1e12}
     if traffic:
         line["pmc_traffic"] = traffic
-    if ens is not None:
-        try:
-            line["ensemble"] = ensemble_bench(ens, steps=2)
-        except Exception as e:  # reported, not hidden
-            line["ensemble"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_grad:
         try:
             from gw_fopt_amd import gradbench
             line["grad"] = gradbench.measure()
         except Exception as e:  # reported, not hidden
             line["grad"] = {"error": f"{type(e).__name__}: {e}"}
+    if ens is not None:
+        try:
+            line["ensemble"] = ensemble_bench(ens, steps=2)
+        except Exception as e:  # reported, not hidden
+            line["ensemble"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb
