@@ -727,8 +727,8 @@ namespace {
 // MODE 0: FP64, one double sincos per (t, omega) term (the parity form).
 // MODE 1: FP64, block-factorised phases: exp(i w psi(t0 + r)) = A(t0) E_r with
 //         E_r = exp(i w r dt (1 - cos th)); double E_r (recurrence), double
-//         block sums; anchors A by an exact FP64 sincos every kReanchor
-//         blocks and A(t0 + RB) = A(t0) E_RB in between.
+//         block sums combined by Horner's rule in E_RB over a chunk's blocks,
+//         one exact anchor sincos per chunk.
 // MODE 2: mixed, the same factorisation and FP64 anchors, with the E_r table,
 //         weights and block sums in float; block results accumulated in float
 //         with Kahan compensation.
@@ -824,10 +824,69 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           }
         }
       } else {
-        // Groups of kReanchor blocks: an exact FP64 sincos gives the anchor
-        // exp(i w psi) of the group's first block; each following block's
-        // anchor is the previous one times E_RB (FP64).  A block's weights are
-        // loaded into registers before its FMA chain so the LDS reads overlap.
+        if constexpr (MODE == 1) {
+          // FP64: Horner over the chunk's blocks, last to first,
+          //   H <- H E_RB + B_b,   C += exp(i w psi(t0)) H,
+          // with one exact FP64 sincos per chunk and omega.  A block's weights
+          // are loaded into registers before its FMA chain so the LDS reads
+          // overlap.
+          double hh[OPL][NS][4];
+#pragma unroll
+          for (int q = 0; q < OPL; ++q)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+              for (int c = 0; c < 4; ++c) hh[q][s][c] = 0.0;
+          const int nb = (len + RB - 1) / RB;
+          for (int b = nb - 1; b >= 0; --b) {
+            const int l0 = b * RB;
+            double vv[RB][2 * NS];  // zero-padded past len
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+              for (int c = 0; c < 2 * NS; ++c) vv[r][c] = L.v[(l0 + r) * 2 * NS + c];
+#pragma unroll
+            for (int q = 0; q < OPL; ++q) {
+#pragma unroll
+              for (int s = 0; s < NS; ++s) {
+                // H E_RB, then the block sum on top
+                double b0 = __builtin_fma(hh[q][s][0], ebr[q], -(hh[q][s][1] * ebi[q]));
+                double b1 = __builtin_fma(hh[q][s][0], ebi[q], hh[q][s][1] * ebr[q]);
+                double b2 = __builtin_fma(hh[q][s][2], ebr[q], -(hh[q][s][3] * ebi[q]));
+                double b3 = __builtin_fma(hh[q][s][2], ebi[q], hh[q][s][3] * ebr[q]);
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                  const double vp = vv[r][2 * s], vx = vv[r][2 * s + 1];
+                  b0 = __builtin_fma(vp, static_cast<double>(er[q][r]), b0);
+                  b1 = __builtin_fma(vp, static_cast<double>(ei[q][r]), b1);
+                  b2 = __builtin_fma(vx, static_cast<double>(er[q][r]), b2);
+                  b3 = __builtin_fma(vx, static_cast<double>(ei[q][r]), b3);
+                }
+                hh[q][s][0] = b0;
+                hh[q][s][1] = b1;
+                hh[q][s][2] = b2;
+                hh[q][s][3] = b3;
+              }
+            }
+          }
+          const double psi0 = L.psi[0];
+#pragma unroll
+          for (int q = 0; q < OPL; ++q) {
+            double as, ac;
+            sincos_f64(om[q] * psi0, &as, &ac);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+              acc[q][s][0] += __builtin_fma(ac, hh[q][s][0], -(as * hh[q][s][1]));
+              acc[q][s][1] += __builtin_fma(ac, hh[q][s][1], as * hh[q][s][0]);
+              acc[q][s][2] += __builtin_fma(ac, hh[q][s][2], -(as * hh[q][s][3]));
+              acc[q][s][3] += __builtin_fma(ac, hh[q][s][3], as * hh[q][s][2]);
+            }
+          }
+        } else {
+        // Mixed: groups of kReanchor blocks; an exact FP64 sincos gives the
+        // anchor exp(i w psi) of the group's first block, each following
+        // block's anchor is the previous one times E_RB (FP64); block sums in
+        // float, rotated by the anchor and added with Kahan compensation.
         for (int g0 = 0; g0 < len; g0 += kReanchor * RB) {
           double anc_r[OPL], anc_i[OPL];
           const double psi0 = L.psi[g0];
@@ -839,10 +898,7 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
 #pragma unroll
             for (int r = 0; r < RB; ++r)
 #pragma unroll
-              for (int c = 0; c < 2 * NS; ++c) {
-                if constexpr (MODE == 2) vv[r][c] = L.vf[(l0 + r) * 2 * NS + c];
-                else vv[r][c] = L.v[(l0 + r) * 2 * NS + c];
-              }
+              for (int c = 0; c < 2 * NS; ++c) vv[r][c] = L.vf[(l0 + r) * 2 * NS + c];
             Acc bs[OPL][NS][4];
 #pragma unroll
             for (int q = 0; q < OPL; ++q)
@@ -876,24 +932,18 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
                 const Acc p1 = fma(ac, bs[q][s][1], as * bs[q][s][0]);
                 const Acc p2 = fma(ac, bs[q][s][2], -as * bs[q][s][3]);
                 const Acc p3 = fma(ac, bs[q][s][3], as * bs[q][s][2]);
-                if constexpr (MODE == 2) {
-                  kahan_add(acc[q][s][0], comp[q][s][0], p0);
-                  kahan_add(acc[q][s][1], comp[q][s][1], p1);
-                  kahan_add(acc[q][s][2], comp[q][s][2], p2);
-                  kahan_add(acc[q][s][3], comp[q][s][3], p3);
-                } else {
-                  acc[q][s][0] += p0;
-                  acc[q][s][1] += p1;
-                  acc[q][s][2] += p2;
-                  acc[q][s][3] += p3;
-                }
+                kahan_add(acc[q][s][0], comp[q][s][0], p0);
+                kahan_add(acc[q][s][1], comp[q][s][1], p1);
+                kahan_add(acc[q][s][2], comp[q][s][2], p2);
+                kahan_add(acc[q][s][3], comp[q][s][3], p3);
               }
-              const double nr = anc_r[q] * ebr[q] - anc_i[q] * ebi[q];
-              const double ni = anc_r[q] * ebi[q] + anc_i[q] * ebr[q];
+              const double nr = __builtin_fma(anc_r[q], ebr[q], -(anc_i[q] * ebi[q]));
+              const double ni = __builtin_fma(anc_r[q], ebi[q], anc_i[q] * ebr[q]);
               anc_r[q] = nr;
               anc_i[q] = ni;
             }
           }
+        }
         }
       }
       __syncthreads();
