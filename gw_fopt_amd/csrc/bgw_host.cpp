@@ -270,10 +270,15 @@ Resolution make_resolution(size_t n_c, size_t n_phi) {
   R.gsin.resize(n_phi);
   for (size_t j = 0; j < n_phi; ++j) {
     R.phi[j] = linspace_at(0.0, 2.0 * kPi, n_phi, j);
-    R.sin_p[j] = std::sin(R.phi[j]);
-    R.cos_p[j] = std::cos(R.phi[j]);
+    // phi.sin() and phi.cos() of one angle in one closure (:228-229), which
+    // the Rust compiler lowers to a single libm sincos call on GNU targets;
+    // sincos and separate sin / cos calls can differ in the last bit.
+    ::sincos(R.phi[j], &R.sin_p[j], &R.cos_p[j]);
     // compute_b_integral (:528-539): trapezoid weights 1/2 at both ends
     const double w = (j == 0 || j == n_phi - 1) ? 0.5 : 1.0;
+    // sin(2 phi) and cos(2 phi) come from two separate passes in the
+    // reference (:528-531): separate libm calls (this file is built without
+    // fast-math, so they are not merged into sincos)
     R.gcos[j] = std::cos(2.0 * R.phi[j]) * w;
     R.gsin[j] = std::sin(2.0 * R.phi[j]) * w;
   }

@@ -113,6 +113,12 @@ inline double linspace_at(double a, double b, size_t n, size_t i) {
   return a + step * static_cast<double>(i);
 }
 
+// libm sin / cos as separate calls: the optimiser may fuse std::sin and
+// std::cos of one argument into sincos; these out-of-line wrappers keep them
+// apart where the reference computes them apart.
+__attribute__((noinline)) double sin_separate(double x) { return std::sin(x); }
+__attribute__((noinline)) double cos_separate(double x) { return std::cos(x); }
+
 struct Cx {
   double re, im;
 };
@@ -120,7 +126,13 @@ inline Cx cmul(Cx a, Cx b) {  // num-complex Mul for Complex
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 inline Cx cscale(Cx a, double s) { return {a.re * s, a.im * s}; }  // Complex * f64
-inline Cx cexp_i(double y) { return {std::cos(y), std::sin(y)}; }  // exp(0 + i y)
+// exp(0 + i y) = from_polar(1, y): cos and sin of one angle together, one
+// libm sincos call as the Rust compiler lowers them.
+inline Cx cexp_i(double y) {
+  double s, c;
+  ::sincos(y, &s, &c);
+  return {c, s};
+}
 
 // ---------------------------------------------------------------------------
 // Lattice bubbles (lattice_bubbles.rs:211-308) and their rotation
@@ -268,8 +280,11 @@ bool set_resolution(Flow& F, size_t n_c, size_t n_phi) {
     double c = F.cos_t[i];
     double s = std::fabs(std::sqrt(1.0 - c * c));
     for (size_t j = 0; j < n_phi; ++j) {
-      double sp = std::sin(F.phi[j]);
-      double cp = std::cos(F.phi[j]);
+      // phi.sin() and phi.cos() of one angle in one closure (:228-229): the
+      // Rust compiler (LLVM) lowers the pair to a single libm sincos call on
+      // GNU targets, whose last bit can differ from separate sin / cos calls.
+      double sp, cp;
+      ::sincos(F.phi[j], &sp, &cp);
       double* x = &F.dirs[(i * n_phi + j) * 4];
       x[0] = 1.0;
       x[1] = s * cp;
@@ -375,10 +390,10 @@ void b_integral(const Flow& F, size_t i, const int32_t* status_row, const double
   for (size_t s = 0; s < F.n_sets; ++s) b_plus[s] = b_minus[s] = 0.0;
   if (delta_ta <= 0.0) return;
   std::vector<double> sin2(n_phi), cos2(n_phi), w(n_phi, 1.0), fac(n_phi), cs(n_phi), cc(n_phi);
-  for (size_t j = 0; j < n_phi; ++j) {
-    sin2[j] = std::sin(2.0 * F.phi[j]);
-    cos2[j] = std::cos(2.0 * F.phi[j]);
-  }
+  // Two separate element-wise passes (azip!, :528-531), i.e. separate sin and
+  // cos calls; kept apart here so the compiler cannot merge them into sincos.
+  for (size_t j = 0; j < n_phi; ++j) sin2[j] = sin_separate(2.0 * F.phi[j]);
+  for (size_t j = 0; j < n_phi; ++j) cos2[j] = cos_separate(2.0 * F.phi[j]);
   w[0] = 0.5;
   w[n_phi - 1] = 0.5;
   const double dta3 = (delta_ta * delta_ta) * delta_ta;  // powi(3)

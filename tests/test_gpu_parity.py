@@ -242,3 +242,32 @@ def test_masks_bitexact_dense_rotated(seed):
             fo, do = oracle.first_colliding(b, ext, 24, 48, a, k_hat=k)
             np.testing.assert_array_equal(f, fo)
             assert d.tobytes() == do.tobytes()
+
+
+@pytest.mark.parametrize("n_phi", [200, 300, 700])
+def test_masks_bitexact_wide_rings(n_phi):
+    """Rings of more than 128 azimuths (sorted in LDS: 256, 512 and 1024 slots)."""
+    inter, ext = small_async(13, n=6)
+    bf = flow(inter, ext, 4, n_phi, [[1.0]], [[5.0]])
+    k = mb.fibonacci_directions(2)[1]
+    for a in range(len(inter)):
+        f, d = bf.first_colliding_in_frame(a, k)
+        fo, do = oracle.first_colliding(inter, ext, 4, n_phi, a, k_hat=k)
+        np.testing.assert_array_equal(f, fo)
+        assert d.tobytes() == do.tobytes()
+    w = np.geomspace(0.5, 40.0, 6)
+    S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision="fp64")
+    So = oracle.spectrum(inter, ext, [k], 4, n_phi, [[1.0]], [[5.0]], w, t_end=0.9, n_t=40)
+    assert rel_per_bin(S, So) <= FP64_REL
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", FP64_REL), ("fp64_blocked", 1e-9)])
+def test_spectrum_many_omegas(precision, bar):
+    """600 omegas: three omegas per lane of a 256-lane workgroup."""
+    wl = configs.plain_8()
+    w = np.geomspace(0.5, 150.0, 600)
+    bf = flow(wl.interior, wl.exterior, 8, 16, [[1.0]], [[5.0]])
+    S, _ = bf.compute_spectrum(w, wl.k_hat[:1], t_end=wl.t_end, n_t=50, precision=precision)
+    So = oracle.spectrum(wl.interior, wl.exterior, wl.k_hat[:1], 8, 16, [[1.0]], [[5.0]], w,
+                         t_end=wl.t_end, n_t=50)
+    assert rel_per_bin(S, So) <= bar
