@@ -48,6 +48,23 @@ class BgwTileConfig(ctypes.Structure):
                 ("items_per_chunk", ctypes.c_size_t), ("persistent_blocks", ctypes.c_size_t)]
 
 
+# BgwLatticeKind, BgwBoundaryCondition, BgwFixedRateMethod
+LATTICE_PARALLELEPIPED, LATTICE_SPHERICAL = 0, 1
+BC_PERIODIC, BC_REFLECTION, BC_NONE = 0, 1, 2
+FIXED_RATE_TIME_STEPPING, FIXED_RATE_DISTRIBUTION = 0, 1
+
+
+class BgwNucleationLattice(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("origin", ctypes.c_double * 3),
+                ("basis", ctypes.c_double * 9), ("radius", ctypes.c_double)]
+
+
+class BgwFixedRateRealisation(ctypes.Structure):
+    _fields_ = [("lattice", BgwNucleationLattice), ("beta", ctypes.c_double),
+                ("gamma0", ctypes.c_double), ("t0", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("stream", ctypes.c_uint64)]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _sp = ctypes.POINTER(ctypes.c_size_t)
@@ -94,6 +111,15 @@ SIGNATURES = {
     "bgw_ensemble_autotune": (
         _st, [_vp, _dp, _sz, _dp, _sz, _dp, _dp, _sz, _st, ctypes.POINTER(BgwTileConfig)]),
     "bgw_ensemble_last_timings": (_st, [_vp, ctypes.POINTER(BgwTimings)]),
+    "bgw_nucleation_images_per_bubble": (_sz, [ctypes.c_int32, _st]),
+    "bgw_nucleation_fixed_rate": (
+        _st, [ctypes.POINTER(BgwFixedRateRealisation), _sz, _st, _st, _d, _sz, _sz, _d, _sz,
+              _dp, _dp, _dp, _dp, _sp]),
+    "bgw_nucleation_solve_distribution": (_st, [_d, _d, _d, _d, _d, _dp, _dp, _sz, _sp]),
+    "bgw_nucleation_spontaneous": (
+        _st, [ctypes.POINTER(BgwNucleationLattice), _st, _sz, _d, ctypes.c_uint64,
+              ctypes.c_uint64, _sz, _dp, _dp]),
+    "bgw_nucleation_last_kernel_ms": (_st, [_dp]),
 }
 
 _lib = None

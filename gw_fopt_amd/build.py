@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libbubble_gw_c.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["bgw_kernels.hip", "bgw_ensemble.hip", "bgw_host.cpp"]
+SOURCES = ["bgw_kernels.hip", "bgw_ensemble.hip", "bgw_nucleation.hip", "bgw_host.cpp"]
 OFFLOAD_ARCH = "gfx950"
 
 

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "bgw_common.h"
+#include "bgw_ffi.h"
 #include "bgw_kernels.h"
 #include "bubble_gw.h"
 
@@ -39,59 +40,8 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;  // std::f64::consts::PI
 constexpr size_t kMaxSetsPerLaunch = 6;
 
-// ---------------------------------------------------------------------------
-// Errors (lib.rs:14-16, 104-134)
-// ---------------------------------------------------------------------------
-thread_local std::string t_last_error;
-
-struct CalcError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-void set_last_error(std::string message) {
-  for (char& c : message)
-    if (c == '\0') c = ' ';
-  t_last_error = std::move(message);
-}
-
-template <class F>
-BgwStatus run_ffi(F&& f) {
-  try {
-    f();
-    t_last_error.clear();
-    return BgwStatus_Ok;
-  } catch (const CalcError& e) {
-    set_last_error(e.what());
-    return BgwStatus_CalculationError;
-  } catch (const std::exception& e) {
-    set_last_error(std::string("C++ exception crossed the C ABI boundary: ") + e.what());
-    return BgwStatus_Panic;
-  } catch (...) {
-    set_last_error("C++ exception crossed the C ABI boundary");
-    return BgwStatus_Panic;
-  }
-}
-
-std::string null_error(const char* name) { return std::string(name) + " must not be null"; }
-
-template <class T>
-const T* slice_from_raw(const T* p, size_t len, const char* name) {
-  if (len == 0) return p;
-  if (!p) throw CalcError(null_error(name));
-  return p;
-}
-
-size_t checked_product(size_t a, size_t b, const char* name) {
-  if (a != 0 && b > std::numeric_limits<size_t>::max() / a)
-    throw CalcError(std::string(name) + " length overflow");
-  return a * b;
-}
-
-void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess)
-    throw CalcError(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
-}
-#define HIPCHECK(x) hip_check((x), #x)
+// Errors (lib.rs:14-16, 104-134): shared with the other host units.
+using namespace bgw_ffi;
 
 // ---------------------------------------------------------------------------
 // Arithmetic shared with the kernels (compiled with -ffp-contract=off)

@@ -1,29 +1,34 @@
 """Python surface of the bulk-flow path, mirroring ``gw_fopt.bubble_gw.many_bubbles``.
 
 Same class names, constructor and method signatures, dtypes, shapes and error
-types as the reference's PyO3 module for this path
-(rust/py-bubble-gw/src/py_many_bubbles/py_generalized_bulk_flow.rs:130-441,
-py_lattice_bubbles.rs:18-140; stub src/gw_fopt/bubble_gw/many_bubbles.pyi):
+types as the reference's PyO3 module (rust/py-bubble-gw/src/py_many_bubbles/:
+py_generalized_bulk_flow.rs:130-441, py_lattice_bubbles.rs, py_lattice.rs,
+py_isometry.rs, py_bubbles_nucleation/; stub src/gw_fopt/bubble_gw/many_bubbles.pyi):
 
-* ``EmptyLattice``, ``ParallelepipedLattice`` (the two lattices the path uses:
-  the C ABI's lattice and the periodic box of the configurations),
+* ``Isometry3``; lattices ``EmptyLattice``, ``ParallelepipedLattice``,
+  ``CartesianLattice``, ``SphericalLattice`` with their exterior images for the
+  periodic, reflection and none boundary conditions,
 * ``LatticeBubbles(lattice, bubbles_interior, bubbles_exterior=None)`` with
-  ``with_boundary_condition("periodic")``,
+  ``with_boundary_condition``, ``transform``, ``translate_time``, ``normalize_time``,
+* ``SpontaneousNucleation``, ``FixedRateNucleation`` (sampled on the device),
 * ``ExponentialTimeCutoff(smax, ratio_t_cut=None, ratio_t_0=None)``,
 * ``GeneralizedBulkFlow(lattice_bubbles)``.
 
 Every computation runs in libbubble_gw_c.so on the GPU; the Python code only
 validates arguments, reshapes arrays and maps errors (InvalidIndex ->
 IndexError, everything else -> ValueError, as py_generalized_bulk_flow.rs:117-126).
+Lattice geometry (containment, images) is evaluated here in the reference's
+operation order.
 
 Additions beyond the reference surface are keyword-only or new methods:
 ``compute_c_integral(..., precision="fp64"|"mixed")``, ``compute_spectrum``,
-``last_timings``.
+``last_timings``, ``nucleate_many`` and ``nucleate_fixed_rate_batch``.
 """
 from __future__ import annotations
 
 import ctypes
 import math
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -31,8 +36,10 @@ import numpy as np
 from . import _lib
 
 __all__ = [
-    "EmptyLattice", "ParallelepipedLattice", "LatticeBubbles", "ExponentialTimeCutoff",
-    "GeneralizedBulkFlow", "fibonacci_directions",
+    "Isometry3", "EmptyLattice", "ParallelepipedLattice", "CartesianLattice", "SphericalLattice",
+    "LatticeBubbles", "ExponentialTimeCutoff", "SpontaneousNucleation", "FixedRateNucleation",
+    "FixedRateNucleationResult", "nucleate_fixed_rate_batch", "GeneralizedBulkFlow",
+    "fibonacci_directions",
 ]
 
 _PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED, "fp64_blocked": _lib.FP64_BLOCKED}
@@ -50,33 +57,224 @@ def _precision(p) -> int:
 
 
 # ---------------------------------------------------------------------------
-# Lattices (rust/bubble-gw/src/many_bubbles/lattice/)
+# Isometries and lattices (rust/bubble-gw/src/many_bubbles/lattice/,
+# rust/py-bubble-gw/src/py_many_bubbles/py_lattice.rs, py_isometry.rs)
 # ---------------------------------------------------------------------------
+_EPS = float(np.finfo(np.float64).eps)
+
+
+def _vec3(v, name="vector") -> np.ndarray:
+    a = np.asarray(v, dtype=np.float64)
+    if a.shape != (3,):
+        raise ValueError(f"{name} must have 3 components")
+    return a.copy()
+
+
+def _mat3(m, name="matrix") -> np.ndarray:
+    a = np.asarray(m, dtype=np.float64)
+    if a.shape != (3, 3):
+        raise ValueError(f"{name} must be 3x3")
+    return a.copy()
+
+
+def _tuple3(v):
+    return tuple(float(x) for x in v)
+
+
+def _euler_matrix(roll, pitch, yaw) -> np.ndarray:
+    """nalgebra Rotation3::from_euler_angles: Rz(yaw) Ry(pitch) Rx(roll)."""
+    sr, cr = math.sin(roll), math.cos(roll)
+    sp, cp = math.sin(pitch), math.cos(pitch)
+    sy, cy = math.sin(yaw), math.cos(yaw)
+    return np.array([[cy * cp, cy * sp * sr - sy * cr, cy * sp * cr + sy * sr],
+                     [sy * cp, sy * sp * sr + cy * cr, sy * sp * cr - cy * sr],
+                     [-sp, cp * sr, cp * cr]])
+
+
+def _nearest_rotation(m) -> np.ndarray:
+    """The rotation closest to m (the reference goes through UnitQuaternion::from_matrix)."""
+    u, _, vt = np.linalg.svd(_mat3(m, "rotation_matrix"))
+    r = u @ vt
+    if np.linalg.det(r) < 0.0:
+        u[:, -1] = -u[:, -1]
+        r = u @ vt
+    return r
+
+
+def _rotation_from_input(euler_angles, rotation_matrix) -> np.ndarray:
+    if euler_angles is not None and rotation_matrix is not None:
+        raise ValueError("Specify either euler_angles or rotation_matrix, not both.")
+    if euler_angles is not None:
+        return _euler_matrix(*_vec3(euler_angles, "euler_angles"))
+    if rotation_matrix is not None:
+        return _nearest_rotation(rotation_matrix)
+    return np.eye(3)
+
+
+class Isometry3:
+    """Rotation followed by translation, x -> R x + t (py_isometry.rs).
+
+    The rotation is held as a 3x3 matrix; the reference holds a unit
+    quaternion, so composed rotations agree to rounding.
+    """
+
+    def __init__(self, translation=(0.0, 0.0, 0.0), *, euler_angles=None, rotation_matrix=None):
+        self._t = _vec3(translation, "translation")
+        self._r = _rotation_from_input(euler_angles, rotation_matrix)
+
+    @classmethod
+    def _from(cls, r, t) -> "Isometry3":
+        iso = cls.__new__(cls)
+        iso._r = np.asarray(r, dtype=np.float64)
+        iso._t = np.asarray(t, dtype=np.float64)
+        return iso
+
+    @staticmethod
+    def identity() -> "Isometry3":
+        return Isometry3()
+
+    @staticmethod
+    def from_translation(translation) -> "Isometry3":
+        return Isometry3(translation)
+
+    @staticmethod
+    def from_rotation(*, euler_angles=None, rotation_matrix=None) -> "Isometry3":
+        return Isometry3(euler_angles=euler_angles, rotation_matrix=rotation_matrix)
+
+    @staticmethod
+    def from_euler_angles(roll: float, pitch: float, yaw: float) -> "Isometry3":
+        return Isometry3(euler_angles=(roll, pitch, yaw))
+
+    @staticmethod
+    def from_axis_angle(axis, angle: float) -> "Isometry3":
+        a = _vec3(axis, "axis")
+        n = float(np.linalg.norm(a))
+        if n < _EPS:
+            raise ValueError("Axis must be a non-zero vector.")
+        k = a / n
+        K = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+        r = np.eye(3) + math.sin(angle) * K + (1.0 - math.cos(angle)) * (K @ K)
+        return Isometry3._from(r, np.zeros(3))
+
+    def transform(self, other: "Isometry3") -> "Isometry3":
+        """self * other: apply other first."""
+        return Isometry3._from(self._r @ other._r, self._r @ other._t + self._t)
+
+    def rotate(self, *, euler_angles=None, rotation_matrix=None) -> "Isometry3":
+        return Isometry3._from(_rotation_from_input(euler_angles, rotation_matrix) @ self._r, self._t)
+
+    def translate(self, translation) -> "Isometry3":
+        return Isometry3._from(self._r, self._t + _vec3(translation, "translation"))
+
+    @property
+    def translation(self):
+        return _tuple3(self._t)
+
+    @property
+    def rotation_matrix(self):
+        return tuple(_tuple3(row) for row in self._r)
+
+    def apply_points(self, points) -> np.ndarray:
+        p = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        return p @ self._r.T + self._t
+
+    def apply_vectors(self, vectors) -> np.ndarray:
+        return np.asarray(vectors, dtype=np.float64).reshape(-1, 3) @ self._r.T
+
+    def __repr__(self):
+        t = self._t
+        return f"Isometry3(translation=[{t[0]:.3f}, {t[1]:.3f}, {t[2]:.3f}], rotation=...)"
+
+
+def _det3_columns(e) -> float:
+    """nalgebra's 3x3 determinant of Matrix3::from_columns(e) (cofactor form)."""
+    m = lambda r, c: float(e[c][r])  # noqa: E731
+    minor_m12_m23 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2)
+    minor_m11_m23 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2)
+    minor_m11_m22 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1)
+    return m(0, 0) * minor_m12_m23 - m(0, 1) * minor_m11_m23 + m(0, 2) * minor_m11_m22
+
+
+def _inverse3_columns(e):
+    """nalgebra's 3x3 try_inverse of Matrix3::from_columns(e); None if singular."""
+    m = lambda r, c: float(e[c][r])  # noqa: E731
+    minor_m12_m23 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2)
+    minor_m11_m23 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2)
+    minor_m11_m22 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1)
+    det = m(0, 0) * minor_m12_m23 - m(0, 1) * minor_m11_m23 + m(0, 2) * minor_m11_m22
+    if det == 0.0:
+        return None
+    return np.array([
+        [minor_m12_m23 / det, (m(0, 2) * m(2, 1) - m(2, 2) * m(0, 1)) / det,
+         (m(0, 1) * m(1, 2) - m(1, 1) * m(0, 2)) / det],
+        [-minor_m11_m23 / det, (m(0, 0) * m(2, 2) - m(2, 0) * m(0, 2)) / det,
+         (m(0, 2) * m(1, 0) - m(1, 2) * m(0, 0)) / det],
+        [minor_m11_m22 / det, (m(0, 1) * m(2, 0) - m(2, 1) * m(0, 0)) / det,
+         (m(0, 0) * m(1, 1) - m(1, 0) * m(0, 1)) / det]])
+
+
+def _check_bc(boundary_condition) -> int:
+    bc = str(boundary_condition).lower()
+    if bc == "periodic":
+        return _lib.BC_PERIODIC
+    if bc == "reflection":
+        return _lib.BC_REFLECTION
+    if bc == "none":
+        return _lib.BC_NONE
+    raise ValueError("Invalid boundary condition. Expected 'periodic', 'reflection', or 'none'.")
+
+
+def _dot3(a, b) -> float:
+    return float(a[0]) * float(b[0]) + float(a[1]) * float(b[1]) + float(a[2]) * float(b[2])
+
+
 class EmptyLattice:
     """No geometry: no containment test, no exterior bubbles (empty_lattice.rs)."""
 
     def name(self) -> str:
         return "EmptyLattice"
 
+    def transform(self, iso: Isometry3) -> "EmptyLattice":
+        return EmptyLattice()
+
     @property
     def volume(self) -> float:
         return 0.0
+
+    def reference_point(self):
+        return (0.0, 0.0, 0.0)
+
+    def parameters(self):
+        return []
 
     def contains(self, points):
         return []
 
     def generate_bubbles_exterior(self, interior, boundary_condition):
+        _check_bc(boundary_condition)
         return np.zeros((0, 4))
+
+    def __repr__(self):
+        return "EmptyLattice"
 
 
 class ParallelepipedLattice:
     """origin + u e1 + v e2 + w e3, 0 <= u, v, w <= 1 (parallelepiped_lattice.rs)."""
 
+    _kind = _lib.LATTICE_PARALLELEPIPED
+
     def __init__(self, origin, basis):
-        self.origin = np.asarray(origin, dtype=np.float64).reshape(3)
-        self.basis = np.asarray(basis, dtype=np.float64).reshape(3, 3)
-        if abs(np.linalg.det(self.basis.T)) <= np.finfo(float).eps:
-            raise ValueError("Invalid ParallelepipedLattice: degenerate basis")
+        self._origin = _vec3(origin, "origin")
+        self._basis = _mat3(basis, "basis")
+        if abs(_det3_columns(self._basis)) <= _EPS:
+            raise ValueError("Basis vectors are linearly dependent (zero volume)")
+
+    @classmethod
+    def _unchecked(cls, origin, basis):
+        obj = cls.__new__(cls)
+        obj._origin = np.asarray(origin, dtype=np.float64)
+        obj._basis = np.asarray(basis, dtype=np.float64)
+        return obj
 
     @staticmethod
     def axis_aligned(origin, lx, ly, lz) -> "ParallelepipedLattice":
@@ -85,34 +283,208 @@ class ParallelepipedLattice:
     @staticmethod
     def cube_centered(center, side) -> "ParallelepipedLattice":
         h = side / 2.0
-        origin = np.asarray(center, dtype=np.float64) - h
+        origin = _vec3(center, "center") - h
         return ParallelepipedLattice(origin, [[side, 0, 0], [0, side, 0], [0, 0, side]])
 
     def name(self) -> str:
         return "ParallelepipedLattice"
 
+    def transform(self, iso: Isometry3):
+        return type(self)._unchecked(iso.apply_points(self._origin)[0], iso.apply_vectors(self._basis))
+
+    @property
+    def origin(self):
+        return _tuple3(self._origin)
+
+    @property
+    def basis(self):
+        return tuple(_tuple3(e) for e in self._basis)
+
+    @property
+    def normalized_basis(self):
+        out = []
+        for e in self._basis:
+            n2 = _dot3(e, e)
+            out.append(_tuple3(e / math.sqrt(n2)) if n2 > _EPS else (0.0, 0.0, 0.0))
+        return tuple(out)
+
     @property
     def volume(self) -> float:
-        return float(abs(np.linalg.det(self.basis.T)))
+        return abs(_det3_columns(self._basis))
+
+    def reference_point(self):
+        return _tuple3(self._origin)
+
+    def parameters(self):
+        return [float(x) for x in (*self._origin, *self._basis.reshape(-1))]
 
     def contains(self, points):
         pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
-        inv = np.linalg.inv(self.basis.T)
-        uvw = (pts - self.origin) @ inv.T
-        return list(np.all((uvw >= 0.0) & (uvw <= 1.0), axis=1))
+        inv = _inverse3_columns(self._basis)
+        if inv is None:
+            return [False] * len(pts)
+        out = []
+        for p in pts:
+            d = p - self._origin
+            ok = True
+            for i in range(3):
+                u = inv[i, 0] * d[0] + inv[i, 1] * d[1] + inv[i, 2] * d[2]
+                ok = ok and 0.0 <= u <= 1.0
+            out.append(bool(ok))
+        return out
+
+    def _lattice_struct(self) -> "_lib.BgwNucleationLattice":
+        st = _lib.BgwNucleationLattice()
+        st.kind = self._kind
+        st.origin[:] = list(self._origin)
+        st.basis[:] = list(self._basis.reshape(-1))
+        st.radius = 0.0
+        return st
 
     def generate_bubbles_exterior(self, interior, boundary_condition):
-        """Periodic: six face images per interior bubble in the order
-        +e1, -e1, +e2, -e2, +e3, -e3 (parallelepiped_lattice.rs:197-215)."""
-        if boundary_condition != "periodic":
-            raise ValueError(f"boundary condition {boundary_condition!r} is not supported")
-        e1, e2, e3 = self.basis
-        shifts = [e1, -e1, e2, -e2, e3, -e3]
-        out = []
-        for ev in np.asarray(interior, dtype=np.float64).reshape(-1, 4):
-            for sh in shifts:
-                out.append([ev[0], ev[1] + sh[0], ev[2] + sh[1], ev[3] + sh[2]])
-        return np.asarray(out, dtype=np.float64).reshape(-1, 4)
+        """Periodic: six face translates per interior bubble in the order +e1, -e1, +e2,
+        -e2, +e3, -e3 (parallelepiped_lattice.rs:197-216); reflection: mirror images
+        in the faces -e1, +e1, -e2, +e2, -e3, +e3 (:217-295); none: no images."""
+        bc = _check_bc(boundary_condition)
+        ev = np.asarray(interior, dtype=np.float64).reshape(-1, 4)
+        if bc == _lib.BC_NONE:
+            return np.zeros((0, 4))
+        out = np.empty((len(ev), 6, 4))
+        out[:, :, 0] = ev[:, None, 0]
+        if bc == _lib.BC_PERIODIC:
+            for j in range(3):
+                out[:, 2 * j, 1:] = ev[:, 1:] + self._basis[j]
+                out[:, 2 * j + 1, 1:] = ev[:, 1:] + (-self._basis[j])
+            return out.reshape(-1, 4)
+        axis = np.eye(3)
+        for j in range(3):
+            e = self._basis[j]
+            nrm = math.sqrt(_dot3(e, e))
+            n = e / nrm if nrm > _EPS else axis[j]
+            q = self._origin + e
+            for b in range(len(ev)):
+                p = ev[b, 1:]
+                nn = -n
+                d0 = _dot3(p - self._origin, nn)
+                out[b, 2 * j, 1:] = p - 2.0 * d0 * nn
+                d1 = _dot3(p - q, n)
+                out[b, 2 * j + 1, 1:] = p - 2.0 * d1 * n
+        return out.reshape(-1, 4)
+
+    def __repr__(self):
+        return f"{self.name()}(origin={self.origin}, basis={self.basis})"
+
+
+class CartesianLattice(ParallelepipedLattice):
+    """A parallelepiped with pairwise orthogonal basis vectors (cartesian_lattice.rs)."""
+
+    def __init__(self, origin, basis):
+        super().__init__(origin, basis)
+        e1, e2, e3 = self._basis
+        eps = 1e-10
+        if abs(_dot3(e1, e2)) > eps or abs(_dot3(e1, e3)) > eps or abs(_dot3(e2, e3)) > eps:
+            raise ValueError("Basis vectors are not pairwise orthogonal")
+
+    @staticmethod
+    def with_origin_and_sizes(origin, sizes) -> "CartesianLattice":
+        lx, ly, lz = (float(x) for x in sizes)
+        return CartesianLattice(origin, [[lx, 0, 0], [0, ly, 0], [0, 0, lz]])
+
+    def name(self) -> str:
+        return "CartesianLattice"
+
+    @property
+    def side_lengths(self):
+        return tuple(math.sqrt(_dot3(e, e)) for e in self._basis)
+
+
+class SphericalLattice:
+    """|x - center| <= radius (spherical_lattice.rs)."""
+
+    _kind = _lib.LATTICE_SPHERICAL
+
+    def __init__(self, center, radius: float):
+        self._center = _vec3(center, "center")
+        self._radius = float(radius)
+
+    def name(self) -> str:
+        return "SphericalLattice"
+
+    def transform(self, iso: Isometry3) -> "SphericalLattice":
+        return SphericalLattice(iso.apply_points(self._center)[0], self._radius)
+
+    @property
+    def center(self):
+        return _tuple3(self._center)
+
+    @property
+    def radius(self) -> float:
+        return self._radius
+
+    @property
+    def volume(self) -> float:
+        r = self._radius
+        return (4.0 / 3.0) * math.pi * (r * r * r)
+
+    def reference_point(self):
+        return _tuple3(self._center)
+
+    def parameters(self):
+        return [*map(float, self._center), self._radius]
+
+    def contains(self, points):
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        r2 = self._radius * self._radius
+        return [bool(_dot3(p - self._center, p - self._center) <= r2) for p in pts]
+
+    def _lattice_struct(self) -> "_lib.BgwNucleationLattice":
+        st = _lib.BgwNucleationLattice()
+        st.kind = self._kind
+        st.origin[:] = list(self._center)
+        st.basis[:] = [0.0] * 9
+        st.radius = self._radius
+        return st
+
+    def generate_bubbles_exterior(self, interior, boundary_condition):
+        """Reflection: the mirror images in the tangent planes where the line through
+        the center and the bubble meets the sphere (spherical_lattice.rs:72-131);
+        periodic and none: no images."""
+        bc = _check_bc(boundary_condition)
+        ev = np.asarray(interior, dtype=np.float64).reshape(-1, 4)
+        if bc != _lib.BC_REFLECTION:
+            return np.zeros((0, 4))
+        c, r = self._center, self._radius
+        out = np.empty((len(ev), 2, 4))
+        for b, e in enumerate(ev):
+            p = e[1:]
+            v = p - c
+            d = math.sqrt(_dot3(v, v))
+            out[b, :, 0] = e[0]
+            if d < _EPS:
+                out[b, 0, 1:] = (c[0] + r, c[1], c[2])
+                out[b, 1, 1:] = (c[0] - r, c[1], c[2])
+                continue
+            u = v / d
+            for side, (q, n) in enumerate(((c + r * u, u), (c - r * u, -u))):
+                dd = _dot3(p - q, n)
+                out[b, side, 1:] = p - 2.0 * dd * n
+        return out.reshape(-1, 4)
+
+    def __repr__(self):
+        return f"SphericalLattice(center={self.center}, radius={self._radius})"
+
+
+_LATTICE_TYPES = (ParallelepipedLattice, CartesianLattice, SphericalLattice, EmptyLattice)
+
+
+def _lattice_type_name(lattice) -> str:
+    if isinstance(lattice, CartesianLattice):
+        return "Cartesian"
+    if isinstance(lattice, ParallelepipedLattice):
+        return "Parallelepiped"
+    if isinstance(lattice, SphericalLattice):
+        return "Spherical"
+    return "Empty"
 
 
 def _bubble_name(i, n_int):
@@ -120,11 +492,13 @@ def _bubble_name(i, n_int):
 
 
 class LatticeBubbles:
-    """Interior (and exterior) bubbles (t, x, y, z) in a lattice (lattice_bubbles.rs:199-365)."""
+    """Interior (and exterior) bubbles (t, x, y, z) in a lattice (lattice_bubbles.rs:199-460,
+    py_lattice_bubbles.rs)."""
 
     def __init__(self, lattice, bubbles_interior, bubbles_exterior=None):
-        if not isinstance(lattice, (EmptyLattice, ParallelepipedLattice)):
-            raise ValueError("Expected a lattice instance: ParallelepipedLattice or EmptyLattice")
+        if not isinstance(lattice, _LATTICE_TYPES):
+            raise ValueError("Expected a lattice instance: ParallelepipedLattice, CartesianLattice, "
+                             "SphericalLattice, or EmptyLattice")
         interior = np.ascontiguousarray(np.asarray(bubbles_interior, dtype=np.float64))
         exterior = (np.zeros((0, 4)) if bubbles_exterior is None
                     else np.ascontiguousarray(np.asarray(bubbles_exterior, dtype=np.float64)))
@@ -132,7 +506,7 @@ class LatticeBubbles:
             raise ValueError(
                 f"Array shape mismatch: Expected 4 columns, got {interior.shape[-1]} for interior, "
                 f"{exterior.shape[-1]} for exterior")
-        if isinstance(lattice, ParallelepipedLattice):
+        if not isinstance(lattice, EmptyLattice):
             inside = lattice.contains(interior[:, 1:])
             bad = [f"Interior({i})" for i, ok in enumerate(inside) if not ok]
             if bad:
@@ -141,10 +515,18 @@ class LatticeBubbles:
             bad = [f"Exterior({i})" for i, ok in enumerate(inside) if ok]
             if bad:
                 raise ValueError(f"Exterior bubbles formed inside lattice: [{', '.join(bad)}]")
-        self.lattice = lattice
+        self._lattice = lattice
         self._interior = interior
         self._exterior = exterior
         self._check_causality()
+
+    @classmethod
+    def _unchecked(cls, lattice, interior, exterior) -> "LatticeBubbles":
+        obj = cls.__new__(cls)
+        obj._lattice = lattice
+        obj._interior = np.ascontiguousarray(interior, dtype=np.float64)
+        obj._exterior = np.ascontiguousarray(exterior, dtype=np.float64).reshape(-1, 4)
+        return obj
 
     def _check_causality(self):
         """check_bubble_formed_inside_bubble (lattice_bubbles.rs:71-119)."""
@@ -161,7 +543,50 @@ class LatticeBubbles:
                     " at initial time (overlapping light cones)")
 
     def with_boundary_condition(self, boundary_condition="periodic") -> None:
-        self._exterior = self.lattice.generate_bubbles_exterior(self._interior, boundary_condition)
+        """Replace the exterior by the lattice's images of the interior (lattice_bubbles.rs:323-365)."""
+        self._exterior = np.ascontiguousarray(
+            self._lattice.generate_bubbles_exterior(self._interior, boundary_condition))
+
+    def transform(self, iso: Isometry3) -> "LatticeBubbles":
+        """Positions x -> R x + t, times unchanged, lattice transformed (lattice_bubbles.rs:144-197)."""
+        def move(b):
+            out = b.copy()
+            if len(b):
+                out[:, 1:] = iso.apply_points(b[:, 1:])
+            return out
+        return LatticeBubbles._unchecked(self._lattice.transform(iso), move(self._interior),
+                                         move(self._exterior))
+
+    def translate_time_mut(self, t_shift: float) -> None:
+        """Add t_shift to every bubble time (lattice_bubbles.rs:392-402)."""
+        self._interior = self._interior.copy()
+        self._exterior = self._exterior.copy()
+        self._interior[:, 0] += t_shift
+        self._exterior[:, 0] += t_shift
+
+    def translate_time(self, t_shift: float) -> "LatticeBubbles":
+        out = LatticeBubbles._unchecked(self._lattice, self._interior, self._exterior)
+        out.translate_time_mut(t_shift)
+        return out
+
+    def normalize_time_mut(self) -> None:
+        """Shift times so the earliest bubble (interior or exterior) is at t = 0 (:439-460)."""
+        times = np.concatenate([self._interior[:, 0], self._exterior[:, 0]])
+        if len(times) and np.isfinite(times.min()):
+            self.translate_time_mut(-float(times.min()))
+
+    def normalize_time(self) -> "LatticeBubbles":
+        out = LatticeBubbles._unchecked(self._lattice, self._interior, self._exterior)
+        out.normalize_time_mut()
+        return out
+
+    @property
+    def lattice(self):
+        return self._lattice
+
+    @property
+    def lattice_type(self) -> str:
+        return _lattice_type_name(self._lattice)
 
     @property
     def bubbles_interior(self):
@@ -173,10 +598,20 @@ class LatticeBubbles:
 
     @property
     def volume(self) -> float:
-        return self.lattice.volume
+        return self._lattice.volume
+
+    @property
+    def reference_point(self):
+        return self._lattice.reference_point()
+
+    def parameters(self):
+        return self._lattice.parameters()
+
+    def contains(self, points):
+        return self._lattice.contains(points)
 
     def __repr__(self):
-        return (f"LatticeBubbles(lattice_type={type(self.lattice).__name__}, "
+        return (f"LatticeBubbles(lattice_type={self.lattice_type}, "
                 f"n_interior={len(self._interior)}, n_exterior={len(self._exterior)})")
 
 
@@ -196,6 +631,253 @@ class ExponentialTimeCutoff:
     def t_0(self) -> float:
         r0 = 0.25 if self.ratio_t_0 is None else self.ratio_t_0
         return r0 * (self.smax - self.t_cut)
+
+
+# ---------------------------------------------------------------------------
+# Nucleation on the device (bubbles_nucleation/*.rs, py_bubbles_nucleation/*.rs)
+# ---------------------------------------------------------------------------
+_METHODS = {"fixed_probability_time_stepping": _lib.FIXED_RATE_TIME_STEPPING,
+            "fixed_probability_distribution": _lib.FIXED_RATE_DISTRIBUTION}
+
+
+def _nucleation_lattice(lattice):
+    if isinstance(lattice, (ParallelepipedLattice, SphericalLattice)):
+        return lattice._lattice_struct()
+    if isinstance(lattice, EmptyLattice):
+        raise ValueError("nucleation needs a lattice with a volume: EmptyLattice has no sample points")
+    raise ValueError("Expected a lattice instance: ParallelepipedLattice, CartesianLattice, "
+                     "SphericalLattice, or EmptyLattice")
+
+
+def _random_seed() -> int:
+    return int.from_bytes(os.urandom(8), "little")
+
+
+def _rust_float(x: float) -> str:
+    return str(int(x)) if float(x).is_integer() and abs(x) < 1e16 else repr(float(x))
+
+
+def _rust_option(x) -> str:
+    return "None" if x is None else f"Some({x})"
+
+
+def _images_per_bubble(lattice, bc: int) -> int:
+    return int(_lib.lib().bgw_nucleation_images_per_bubble(lattice._kind, bc))
+
+
+class SpontaneousNucleation:
+    """n_bubbles bubbles at t0, uniform in the lattice (spontaneous_nucleation.rs,
+    py_spontaneous_nucleation.rs), sampled on the device.
+
+    Random numbers come from Philox4x64-10 with key (seed, stream); each call
+    to ``nucleate`` uses the next stream, as the reference's generator advances
+    between calls.
+    """
+
+    def __init__(self, n_bubbles: int, t0: float = 0.0, seed: int | None = None):
+        self._n_bubbles = int(n_bubbles)
+        self._t0 = float(t0)
+        self._seed = seed
+        self._key = int(seed) if seed is not None else _random_seed()
+        self._stream = 0
+
+    @property
+    def n_bubbles(self) -> int:
+        return self._n_bubbles
+
+    @property
+    def t0(self) -> float:
+        return self._t0
+
+    @property
+    def seed(self):
+        return self._seed
+
+    def __repr__(self):
+        return (f"SpontaneousNucleation(n_bubbles={self._n_bubbles}, t0={_rust_float(self._t0)}, "
+                f"seed={_rust_option(self._seed)})")
+
+    def nucleate_many(self, lattice, n_realisations: int, boundary_condition="periodic"):
+        """Interior (n_real, n_bubbles, 4) and exterior (n_real, n_bubbles * n_img, 4) of
+        n_realisations independent realisations, in one launch."""
+        bc = _check_bc(boundary_condition)
+        st = _nucleation_lattice(lattice)
+        n_img = _images_per_bubble(lattice, bc)
+        n_real = int(n_realisations)
+        interior = np.zeros((n_real, self._n_bubbles, 4))
+        exterior = np.zeros((n_real, self._n_bubbles * n_img, 4))
+        status = _lib.lib().bgw_nucleation_spontaneous(
+            ctypes.byref(st), bc, self._n_bubbles, self._t0, ctypes.c_uint64(self._key & (2**64 - 1)),
+            ctypes.c_uint64(self._stream), n_real, _lib.dptr(interior),
+            _lib.dptr(exterior) if n_img else None)
+        if status != _lib.OK:
+            raise ValueError(_lib.last_error())
+        self._stream += n_real
+        return interior, exterior
+
+    def nucleate(self, lattice, boundary_condition="periodic") -> LatticeBubbles:
+        interior, exterior = self.nucleate_many(lattice, 1, boundary_condition)
+        return LatticeBubbles(lattice, interior[0], exterior[0])
+
+
+@dataclass
+class FixedRateNucleationResult:
+    """lattice_bubbles, time_history and volume_false_vacuum_history of one run."""
+    lattice_bubbles: LatticeBubbles
+    time_history: np.ndarray
+    volume_false_vacuum_history: np.ndarray
+
+
+def nucleate_fixed_rate_batch(specs, *, boundary_condition="periodic",
+                              method="fixed_probability_distribution", d_p0: float = 0.01,
+                              n_points: int = 10000, max_time_iterations: int | None = None,
+                              cutoff_fraction_false_vacuum: float | None = None,
+                              max_bubbles: int | None = None):
+    """Fixed-rate nucleation of many realisations in one device launch.
+
+    ``specs``: a sequence of (lattice, beta, gamma0, t0, seed, stream); every
+    lattice must be of one kind.  Returns (interior, exterior, time_history,
+    volume_history, counts) with interior (n_real, max_bubbles, 4), exterior
+    (n_real, max_bubbles * n_img, 4), histories (n_real, max_bubbles); row
+    counts[r] and beyond are unused.  ``max_bubbles`` defaults to n_points,
+    which no realisation can exceed.
+    """
+    bc = _check_bc(boundary_condition)
+    m = _METHODS.get(str(method).lower())
+    if m is None:
+        raise ValueError("method must be 'fixed_probability_time_stepping' or "
+                         "'fixed_probability_distribution'")
+    specs = list(specs)
+    n_real = len(specs)
+    reals = (_lib.BgwFixedRateRealisation * max(n_real, 1))()
+    n_img = None
+    for r, (lattice, beta, gamma0, t0, seed, stream) in enumerate(specs):
+        reals[r].lattice = _nucleation_lattice(lattice)
+        reals[r].beta, reals[r].gamma0, reals[r].t0 = float(beta), float(gamma0), float(t0)
+        reals[r].seed = int(seed) & (2**64 - 1)
+        reals[r].stream = int(stream) & (2**64 - 1)
+        k = _images_per_bubble(lattice, bc)
+        if n_img is not None and k != n_img:
+            raise ValueError("every realisation must use the same lattice kind")
+        n_img = k
+    n_img = n_img or 0
+    cap = int(n_points if max_bubbles is None else max_bubbles)
+    max_it = 1_000_000 if max_time_iterations is None else int(max_time_iterations)
+    cutoff = 0.01 if cutoff_fraction_false_vacuum is None else float(cutoff_fraction_false_vacuum)
+    interior = np.zeros((n_real, cap, 4))
+    exterior = np.zeros((n_real, cap * n_img, 4))
+    t_hist = np.zeros((n_real, cap))
+    v_hist = np.zeros((n_real, cap))
+    counts = np.zeros(n_real, dtype=np.uintp)
+    status = _lib.lib().bgw_nucleation_fixed_rate(
+        reals, n_real, bc, m, float(d_p0), int(n_points), max_it, cutoff, cap,
+        _lib.dptr(interior), _lib.dptr(exterior) if n_img else None, _lib.dptr(t_hist),
+        _lib.dptr(v_hist), _lib.sptr(counts))
+    if status != _lib.OK:
+        raise ValueError(_lib.last_error())
+    return interior, exterior, t_hist, v_hist, counts.astype(np.int64)
+
+
+def last_nucleation_kernel_ms() -> float:
+    """Device time of the most recent fixed-rate nucleation kernel on this thread."""
+    out = ctypes.c_double(0.0)
+    _lib.check(_lib.lib().bgw_nucleation_last_kernel_ms(ctypes.byref(out)))
+    return out.value
+
+
+class FixedRateNucleation:
+    """Monte Carlo nucleation at rate gamma0 exp(beta (t - t0)) per unit volume of false
+    vacuum (fixed_rate_nucleation.rs, py_fixed_rate_nucleation.rs), on the device.
+
+    The false-vacuum volume is the fraction of n_points sample points that no
+    bubble (or exterior image) has reached; a new bubble nucleates at the first
+    sample point still in the false vacuum.  Random numbers come from
+    Philox4x64-10 with key (seed, stream); each realisation uses the next stream.
+    """
+
+    def __init__(self, beta: float, gamma0: float, t0: float, d_p0: float, seed: int | None = None,
+                 n_points: int = 10000, max_time_iterations: int | None = None,
+                 cutoff_fraction_false_vacuum: float | None = None):
+        if any(math.isnan(float(x)) for x in (beta, gamma0, t0, d_p0)):
+            raise ValueError("Parameters must be finite")
+        if d_p0 <= 0.0 or d_p0 > 1.0:
+            raise ValueError("d_p0 must be in (0, 1]")
+        if gamma0 < 0.0:
+            raise ValueError("gamma0 must be non-negative")
+        if int(n_points) == 0:
+            raise ValueError("n_points must be > 0")
+        self._beta, self._gamma0, self._t0 = float(beta), float(gamma0), float(t0)
+        self._seed = seed
+        self._key = int(seed) if seed is not None else _random_seed()
+        self._stream = 0
+        self._n_points = int(n_points)
+        self._max_it = 1_000_000 if max_time_iterations is None else int(max_time_iterations)
+        self._cutoff = 0.01 if cutoff_fraction_false_vacuum is None else float(cutoff_fraction_false_vacuum)
+
+    @property
+    def beta(self) -> float:
+        return self._beta
+
+    @property
+    def gamma0(self) -> float:
+        return self._gamma0
+
+    @property
+    def t0(self) -> float:
+        return self._t0
+
+    @property
+    def seed(self):
+        return self._seed
+
+    @property
+    def n_points(self) -> int:
+        return self._n_points
+
+    def __repr__(self):
+        return (f"FixedRateNucleation(beta={_rust_float(self._beta)}, gamma0={_rust_float(self._gamma0)}, "
+                f"t0={_rust_float(self._t0)}, seed={_rust_option(self._seed)}, n_points={self._n_points})")
+
+    def solve_bubbles_distribution(self, taumax: float, volume_lattice: float, rtol: float = 1e-9,
+                                   atol: float = 1e-12):
+        """(t, n): the points of the even grid of step 0.001 in tau = beta t where the
+        expected bubble count n crosses an integer (fixed_rate_nucleation.rs:144-186).
+        The integrator is fourth-order Runge-Kutta on that grid (error far below
+        rtol and atol, which are accepted for the reference's signature)."""
+        L = _lib.lib()
+        n = ctypes.c_size_t(0)
+        args = (float(self._beta), float(self._gamma0), float(self._t0), float(taumax),
+                float(volume_lattice))
+        _lib.check(L.bgw_nucleation_solve_distribution(*args, None, None, 0, ctypes.byref(n)))
+        t = np.zeros(n.value)
+        nb = np.zeros(n.value)
+        _lib.check(L.bgw_nucleation_solve_distribution(*args, _lib.dptr(t), _lib.dptr(nb), n.value,
+                                                       ctypes.byref(n)))
+        return t, nb
+
+    def nucleate_many(self, lattice, n_realisations: int, boundary_condition="periodic",
+                      method="fixed_probability_distribution", d_p0: float = 0.01,
+                      max_bubbles: int | None = None):
+        """n_realisations independent runs in one device launch: (interior, exterior,
+        time_history, volume_history, counts) as ``nucleate_fixed_rate_batch``."""
+        n_real = int(n_realisations)
+        specs = [(lattice, self._beta, self._gamma0, self._t0, self._key, self._stream + r)
+                 for r in range(n_real)]
+        out = nucleate_fixed_rate_batch(
+            specs, boundary_condition=boundary_condition, method=method, d_p0=d_p0,
+            n_points=self._n_points, max_time_iterations=self._max_it,
+            cutoff_fraction_false_vacuum=self._cutoff, max_bubbles=max_bubbles)
+        self._stream += n_real
+        return out
+
+    def nucleate(self, lattice, boundary_condition="periodic", method="fixed_probability_distribution",
+                 d_p0: float = 0.01) -> FixedRateNucleationResult:
+        interior, exterior, t_hist, v_hist, counts = self.nucleate_many(
+            lattice, 1, boundary_condition, method, d_p0)
+        n = int(counts[0])
+        n_img = exterior.shape[1] // max(interior.shape[1], 1)
+        lb = LatticeBubbles(lattice, interior[0, :n], exterior[0, :n * n_img])
+        return FixedRateNucleationResult(lb, t_hist[0, :n].copy(), v_hist[0, :n].copy())
 
 
 def fibonacci_directions(n: int) -> np.ndarray:

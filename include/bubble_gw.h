@@ -103,6 +103,50 @@ typedef struct BgwTileConfig {
                                  device at once) */
 } BgwTileConfig;
 
+/* ---- additions: nucleation types ------------------------------------------ */
+
+/* Lattice of a nucleation run (rust/bubble-gw/src/many_bubbles/lattice/):
+ * Parallelepiped (also the Cartesian box): origin + u e1 + v e2 + w e3,
+ * 0 <= u, v, w <= 1, basis rows e1, e2, e3; Spherical: |x - origin| <= radius. */
+typedef enum BgwLatticeKind {
+    BgwLatticeKind_Parallelepiped = 0,
+    BgwLatticeKind_Spherical = 1,
+} BgwLatticeKind;
+
+/* BoundaryConditions (lattice.rs): exterior images of every interior bubble.
+ * Parallelepiped: Periodic = six face translates (+e1, -e1, +e2, -e2, +e3, -e3),
+ * Reflection = six face mirror images (-e1, +e1, -e2, +e2, -e3, +e3 faces);
+ * Spherical: Reflection = two tangent-plane mirror images, Periodic = none. */
+typedef enum BgwBoundaryCondition {
+    BgwBoundaryCondition_Periodic = 0,
+    BgwBoundaryCondition_Reflection = 1,
+    BgwBoundaryCondition_None = 2,
+} BgwBoundaryCondition;
+
+typedef struct BgwNucleationLattice {
+    int32_t kind;       /* BgwLatticeKind */
+    double origin[3];   /* parallelepiped origin / sphere center */
+    double basis[9];    /* parallelepiped basis rows e1, e2, e3 */
+    double radius;      /* sphere radius */
+} BgwNucleationLattice;
+
+/* FixedRateNucleationMethod (fixed_rate_nucleation.rs:35-40). */
+typedef enum BgwFixedRateMethod {
+    BgwFixedRateMethod_TimeStepping = 0,  /* fixed-probability time stepping with d_p0 */
+    BgwFixedRateMethod_Distribution = 1,  /* nucleation times from the expected-count ODE */
+} BgwFixedRateMethod;
+
+/* One fixed-rate realisation: Gamma(t) = gamma0 exp(beta (t - t0)).  Its random
+ * stream is Philox4x64-10 with key (seed, stream). */
+typedef struct BgwFixedRateRealisation {
+    BgwNucleationLattice lattice;
+    double beta;
+    double gamma0;
+    double t0;
+    uint64_t seed;
+    uint64_t stream;
+} BgwFixedRateRealisation;
+
 #ifdef __cplusplus
 extern "C" {
 #endif  // __cplusplus
@@ -304,6 +348,56 @@ BgwStatus bgw_ensemble_autotune(bgw_ensemble *ensemble, const double *w_arr, siz
                                 BgwTileConfig *best);
 
 BgwStatus bgw_ensemble_last_timings(bgw_ensemble *ensemble, BgwTimings *out);
+
+/* ---- additions: nucleation on the device ------------------------------------ */
+
+/* Exterior images per interior bubble for a lattice kind and boundary condition. */
+size_t bgw_nucleation_images_per_bubble(int32_t lattice_kind, BgwBoundaryCondition boundary_condition);
+
+/* Reference: FixedRateNucleation::nucleate (fixed_rate_nucleation.rs:358-377) with
+ * nucleate_fixed_probability_time_stepping (:188-275) or
+ * nucleate_fixed_probability_distribution (:277-356), for n_realisations
+ * independent realisations in one launch (one workgroup each).  Shared
+ * parameters: d_p0 (time stepping), n_points (Monte Carlo sample points,
+ * > 0), max_time_iterations, cutoff_fraction (false-vacuum fraction that ends
+ * the run).  Realisation r writes at most max_bubbles bubbles:
+ * interior rows [r * max_bubbles, ...) of (n_real * max_bubbles, 4),
+ * exterior rows [r * max_bubbles * n_img, ...) of (n_real * max_bubbles * n_img, 4)
+ * (images of bubble b at b * n_img, in generate_bubbles_exterior order),
+ * time_history and volume_history (n_real * max_bubbles) at each nucleation,
+ * counts[r] = bubbles nucleated.  max_bubbles >= n_points can never overflow
+ * (every bubble takes a distinct sample point); a realisation that would
+ * overflow fails the call.  exterior may be NULL when n_img == 0. */
+BgwStatus bgw_nucleation_fixed_rate(const BgwFixedRateRealisation *realisations,
+                                    size_t n_realisations,
+                                    BgwBoundaryCondition boundary_condition,
+                                    BgwFixedRateMethod method, double d_p0, size_t n_points,
+                                    size_t max_time_iterations, double cutoff_fraction,
+                                    size_t max_bubbles, double *interior, double *exterior,
+                                    double *time_history, double *volume_history,
+                                    size_t *counts);
+
+/* Reference: FixedRateNucleation::solve_bubbles_distribution
+ * (fixed_rate_nucleation.rs:144-186): integrates the expected-count ODE from
+ * tau0 = t0 beta to taumax on the even grid of step 0.001 in tau = beta t and
+ * returns the grid points where floor(n) increases (t = tau / beta) with n
+ * there.  With t_out == NULL only *n_len is set; otherwise up to cap entries
+ * are written. */
+BgwStatus bgw_nucleation_solve_distribution(double beta, double gamma0, double t0, double taumax,
+                                            double volume_lattice, double *t_out, double *n_out,
+                                            size_t cap, size_t *n_len);
+
+/* Reference: SpontaneousNucleation::nucleate (spontaneous_nucleation.rs:89-104):
+ * n_bubbles bubbles at t0, uniform in the lattice, for n_realisations
+ * realisations (realisation r uses key (seed, stream0 + r)).  interior:
+ * (n_real * n_bubbles, 4); exterior: (n_real * n_bubbles * n_img, 4). */
+BgwStatus bgw_nucleation_spontaneous(const BgwNucleationLattice *lattice,
+                                     BgwBoundaryCondition boundary_condition, size_t n_bubbles,
+                                     double t0, uint64_t seed, uint64_t stream0,
+                                     size_t n_realisations, double *interior, double *exterior);
+
+/* Device time of the most recent bgw_nucleation_fixed_rate kernel on this thread (ms). */
+BgwStatus bgw_nucleation_last_kernel_ms(double *out);
 
 #ifdef __cplusplus
 }  // extern "C"

@@ -34,7 +34,8 @@ _i = ctypes.c_int
 
 def build(force: bool = False) -> str:
     """Compile the oracle with ``make`` (g++ -O2 -fopenmp -ffp-contract=off)."""
-    if force or not os.path.exists(_LIB_PATH):
+    libs = (_LIB_PATH, os.path.join(_HERE, "liboracle_nucleation.so"))
+    if force or not all(os.path.exists(p) for p in libs):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
     return _LIB_PATH
 
