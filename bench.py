@@ -18,7 +18,8 @@ points/s of the fastest form whose largest per-bin relative error is
 stream, bubbles resident in HBM).
 
 Also reported: the batched ensemble (512 realisations, uneven bubble counts,
-one graph-captured launch with autotuned tiles), the differentiation workload
+one graph-captured launch with autotuned tiles), device nucleation of 512
+fixed-rate realisations in one launch, the differentiation workload
 (adjoint gradients per second, finite-difference check, 200-step fit), the
 roofline of the phase-sum kernel, its HBM traffic from rocprofv3 counters,
 and the CPU oracle on a bounded sample.
@@ -55,6 +56,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-grad", action="store_true")
     p.add_argument("--no-ensemble", action="store_true")
+    p.add_argument("--no-nucleation", action="store_true")
+    p.add_argument("--profile-nucleation", action="store_true",
+                   help="only run the 512-realisation nucleation batch --steps times (for rocprofv3)")
     p.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic counters")
     p.add_argument("--modes", default="mixed,fp64_blocked,fp64")
     p.add_argument("--profile-mode", default=None,
@@ -272,6 +276,71 @@ def ensemble_bench(ens, steps):
                                      "std_S_first_bin": float(ref["std"][0, 0])}}
 
 
+NUC_KW = dict(boundary_condition="periodic", method="fixed_probability_time_stepping", d_p0=0.01,
+              n_points=10000, max_bubbles=4096)
+
+
+def nucleation_specs(n_real=512, seed=11):
+    """512 fixed-rate realisations in the unit Cartesian box, beta ~ U[10, 30],
+    Gamma(t) = 0.01 beta^4 exp(beta t) (gamma0 beta^-4 = 0.01), seed 11, stream r."""
+    import numpy as np
+    from gw_fopt_amd import many_bubbles as mb
+    rng = np.random.default_rng(seed)
+    lat = mb.CartesianLattice((0.0, 0.0, 0.0), np.eye(3))
+    return [(lat, float(b), 0.01 * float(b) ** 4, 0.0, seed, r)
+            for r, b in enumerate(rng.uniform(10.0, 30.0, n_real))]
+
+
+def nucleation_bench(steps=2, cpu_sample=4):
+    """Device nucleation of 512 realisations (one launch) and the oracle on a sample."""
+    import numpy as np
+    from gw_fopt_amd import many_bubbles as mb
+    specs = nucleation_specs()
+    mb.nucleate_fixed_rate_batch(specs[:8], **NUC_KW)  # load and warm up
+    ms, walls = [], []
+    for _ in range(max(1, steps)):
+        t0 = time.perf_counter()
+        interior, exterior, th, vh, counts = mb.nucleate_fixed_rate_batch(specs, **NUC_KW)
+        walls.append(time.perf_counter() - t0)
+        ms.append(mb.last_nucleation_kernel_ms())
+    kernel_ms = float(np.median(ms))
+    out = {"workload": "nucleation_512: 512 fixed-rate realisations (time stepping, d_p0 0.01, "
+                       "10000 sample points, cutoff 1 %), unit Cartesian box, periodic, "
+                       "beta ~ U[10,30], gamma0 = 0.01 beta^4; one launch",
+           "kernel_ms": kernel_ms, "wall_s": float(np.median(walls)),
+           "realisations_per_s": len(specs) / (kernel_ms * 1e-3),
+           "bubbles": int(counts.sum()), "bubbles_per_s": float(counts.sum()) / (kernel_ms * 1e-3),
+           "bubble_counts": [int(counts.min()), int(counts.max())]}
+    try:
+        import oracle.nucleation as on
+        rng = np.random.default_rng(5)
+        sample = sorted(rng.choice(len(specs), cpu_sample, replace=False).tolist())
+        t0 = time.perf_counter()
+        same = True
+        for r in sample:
+            _, beta, gamma0, t0_, seed, stream = specs[r]
+            oi, _, _, ov = on.fixed_rate(("box", [0.0, 0.0, 0.0], np.eye(3)), 0, 0, beta=beta,
+                                         gamma0=gamma0, t0=t0_, d_p0=NUC_KW["d_p0"], seed=seed,
+                                         stream=stream, n_points=NUC_KW["n_points"])
+            n = int(counts[r])
+            same = same and n == len(oi) and np.array_equal(interior[r, :n, 1:], oi[:, 1:]) \
+                and np.array_equal(vh[r, :n], ov)
+        cpu_s = time.perf_counter() - t0
+        share = float(counts[sample].sum()) / float(counts.sum())
+        out["cpu_baseline"] = {
+            "kind": "port", "cores": 1, "sample": f"realisations {sample} (beta "
+            f"{[round(specs[r][1], 2) for r in sample]}) on the nucleation oracle, one thread, "
+            f"{cpu_s:.1f} s; scaled to the whole batch by bubble count (the oracle's work is "
+            "proportional to the iterations, about bubbles / d_p0)",
+            "sample_s": cpu_s, "batch_s_extrapolated": cpu_s / share,
+            "realisations_per_s": len(specs) * share / cpu_s}
+        out["speedup_vs_cpu_baseline"] = out["realisations_per_s"] / out["cpu_baseline"]["realisations_per_s"]
+        out["sample_matches_oracle"] = bool(same)
+    except Exception as e:  # reported, not hidden
+        out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
+    return out
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -285,6 +354,14 @@ def main():
     import numpy as np
     from gw_fopt_amd import configs
 
+    if args.profile_nucleation:
+        from gw_fopt_amd import many_bubbles as mb
+        specs = nucleation_specs()
+        for _ in range(args.steps):
+            _, _, _, _, counts = mb.nucleate_fixed_rate_batch(specs, **NUC_KW)
+        print(json.dumps({"profile": "nucleation_512", "kernel_ms": mb.last_nucleation_kernel_ms(),
+                          "bubbles": int(counts.sum())}), flush=True)
+        return
     wl = configs.plain_256(n_k=args.n_k)
     if args.profile_mode:
         from gw_fopt_amd import many_bubbles as mb
@@ -391,6 +468,11 @@ def main():
             line["ensemble"] = ensemble_bench(ens, steps=2)
         except Exception as e:  # reported, not hidden
             line["ensemble"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_nucleation:
+        try:
+            line["nucleation"] = nucleation_bench()
+        except Exception as e:  # reported, not hidden
+            line["nucleation"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb
