@@ -614,6 +614,57 @@ class LatticeBubbles:
         return (f"LatticeBubbles(lattice_type={self.lattice_type}, "
                 f"n_interior={len(self._interior)}, n_exterior={len(self._exterior)})")
 
+    # -- CSV files: rows t,x,y,z (lattice_bubbles.rs:468-542) --------------------------
+    @staticmethod
+    def load_bubbles_from_csv(path, has_headers: bool = False) -> np.ndarray:
+        """(n, 4) float64 from a comma-separated file; every row must have 4 fields."""
+        import csv
+        with open(path, newline="") as fh:
+            rows = list(csv.reader(fh))
+        if has_headers and rows:
+            rows = rows[1:]
+        rows = [r for r in rows if r]
+        width = {len(r) for r in rows}
+        if len(width) > 1:
+            raise ValueError("deserialize_array2 error")
+        if rows and width != {4}:
+            raise ValueError("Invalid number of columns")
+        try:
+            arr = np.array([[float(v) for v in r] for r in rows], dtype=np.float64)
+        except ValueError:
+            raise ValueError("deserialize_array2 error") from None
+        return arr.reshape(-1, 4)
+
+    @classmethod
+    def from_csv_files(cls, interior_path, exterior_path, lattice,
+                       has_headers: bool = False) -> "LatticeBubbles":
+        return cls(lattice, cls.load_bubbles_from_csv(interior_path, has_headers),
+                   cls.load_bubbles_from_csv(exterior_path, has_headers))
+
+    @staticmethod
+    def _write_csv(bubbles, path, has_headers: bool) -> None:
+        def fmt(x: float) -> str:
+            r = repr(float(x))  # shortest representation that reads back exactly
+            if "e" in r:
+                mant, exp = r.split("e")
+                r = f"{mant}e{int(exp)}"
+            return r
+        with open(path, "w", newline="") as fh:
+            if has_headers:
+                fh.write("t,x,y,z\n")
+            for row in np.asarray(bubbles, dtype=np.float64).reshape(-1, 4):
+                fh.write(",".join(fmt(v) for v in row) + "\n")
+
+    def write_bubbles_interior_to_csv(self, path, has_headers: bool = False) -> None:
+        self._write_csv(self._interior, path, has_headers)
+
+    def save_bubbles_exterior_to_csv(self, path, has_headers: bool = False) -> None:
+        self._write_csv(self._exterior, path, has_headers)
+
+    def save_to_csv_files(self, interior_path, exterior_path, has_headers: bool = False) -> None:
+        self.write_bubbles_interior_to_csv(interior_path, has_headers)
+        self.save_bubbles_exterior_to_csv(exterior_path, has_headers)
+
 
 @dataclass
 class ExponentialTimeCutoff:

@@ -271,3 +271,31 @@ def test_spectrum_many_omegas(precision, bar):
     So = oracle.spectrum(wl.interior, wl.exterior, wl.k_hat[:1], 8, 16, [[1.0]], [[5.0]], w,
                          t_end=wl.t_end, n_t=50)
     assert rel_per_bin(S, So) <= bar
+
+
+def test_reference_example_configuration_conf_y():
+    """The reference's many-bubble example input (docs/examples/inputs/confY.txt, 48
+    bubbles, periodic box L = 2; docs/examples/generalized_bulkflow.py:741-790): first
+    colliders and the collision status at t = 1 bit-exact, C and S within 1e-10."""
+    from test_lattice_io import conf_y_bubbles
+    lb = conf_y_bubbles()
+    inter, ext = lb.bubbles_interior, lb.bubbles_exterior
+    bf = mb.GeneralizedBulkFlow(lb)
+    bf.set_resolution(24, 48)
+    bf.set_gradient_scaling_params([[1.0]], [[5.0]])
+    for a in (0, 17, 47):
+        first = bf.compute_first_colliding_bubble(a)
+        fo, dto = oracle.first_colliding(inter, ext, 24, 48, a)
+        np.testing.assert_array_equal(first, fo)
+        dt = bf.compute_delta_tab(a, first)
+        assert dt.tobytes() == dto.tobytes()
+        st = bf.compute_collision_status(a, 1.0, first, dt)
+        np.testing.assert_array_equal(st, oracle.collision_status(inter[a, 0], 1.0, fo, dto))
+    w = np.geomspace(0.5, 50.0, 12)
+    C = bf.compute_c_integral(w, t_end=2.0, n_t=150)
+    Co = oracle.compute_c_integral(inter, ext, 24, 48, [[1.0]], [[5.0]], w, t_end=2.0, n_t=150)
+    assert rel_to_max(C, Co) <= FP64_REL
+    k = mb.fibonacci_directions(3)
+    S, _ = bf.compute_spectrum(w, k, t_end=2.0, n_t=150)
+    So = oracle.spectrum(inter, ext, k, 24, 48, [[1.0]], [[5.0]], w, t_end=2.0, n_t=150)
+    assert rel_per_bin(S, So) <= FP64_REL
