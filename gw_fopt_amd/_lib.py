@@ -91,6 +91,10 @@ SIGNATURES = {
     "bgw_generalized_bulk_flow_first_colliding_bubble_ex": (_st, [_vp, _sz, _dp, _ip, _dp, _sz]),
     "bgw_generalized_bulk_flow_delta_tab": (_st, [_vp, _sz, _ip, _sz, _dp, _sz]),
     "bgw_generalized_bulk_flow_collision_status": (_st, [_vp, _sz, _d, _ip, _dp, _sz, _ip, _sz]),
+    "bgw_generalized_bulk_flow_compute_c_tensor": (
+        _st, [_vp, _dp, _sz, _b, _d, _d, _sz, _sp, _sz, _st, _dp, _sz]),
+    "bgw_generalized_bulk_flow_compute_c_tensor_integrand": (
+        _st, [_vp, _dp, _sz, _b, _d, _d, _sz, _sp, _sz, _dp, _sz]),
     "bgw_generalized_bulk_flow_compute_c_integrand": (
         _st, [_vp, _dp, _sz, _b, _d, _d, _sz, _sp, _sz, _dp, _sz]),
     "bgw_generalized_bulk_flow_compute_spectrum": (

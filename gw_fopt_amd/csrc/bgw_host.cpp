@@ -1241,7 +1241,8 @@ void check_unit_vectors(const double* k_hat, size_t n_k) {
 std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr, size_t n_w,
                               bool has_t_begin, double t_begin, double t_end, size_t n_t,
                               const size_t* selected, size_t n_selected,
-                              BgwTimeCutoffConfig cutoff, int mode) {
+                              BgwTimeCutoffConfig cutoff, int mode,
+                              const Resolution* surface = nullptr) {
   slice_from_raw(w_arr, n_w, "w_arr");
   const size_t n_sets = h->sc.n_sets;
   std::vector<double> zeros(2 * n_sets * n_w * 2, 0.0);
@@ -1251,7 +1252,7 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
   if (ids.empty()) return zeros;
   const double tb = has_t_begin ? t_begin : 0.0;
   check_time_range(tb, t_end);
-  const Resolution& res = need_resolution(h->res);
+  const Resolution& res = surface ? *surface : need_resolution(h->res);
   Request q;
   q.res = &res;
   q.sc = &h->sc;
@@ -1266,9 +1267,89 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
   q.bubbles_of_real = {ids};
   q.mode = mode;
   q.want_s = false;
+  if (surface) q.skip_uncollided = false;  // components such as xz may be small
   apply_tiles(h->tiles, q);
   Result r = run(h->engine, q);
   return r.c;
+}
+
+// Surface weights of the tensor components (new_generalized_bulk_flow.rs:592-727):
+// the phi-integral of component c on ring i is
+//   sum_j w_j f_j phi_c(phi_j) * theta_c(theta_i) * dphi * dta^3,
+// phi_c in {cos^2, cos sin, cos, sin^2, sin, 1}, theta_c in {sin^2, sin^2,
+// sin cos, sin^2, sin cos, cos^2}, with sin = sqrt(max(1 - cos^2, 0)).  The
+// phase sums take two phi-functions with one theta-factor per launch, so the
+// six components are four launches: (xx, xy), (xz, yz), (yy, -), (zz, -).
+// The integrand path (compute_c_integrand, generalized_bulk_flow.rs:875-920) on the
+// surface weights of `surface` (the handle's own when null).  The two components
+// of the run are written to components dst[0], dst[1] (skipped when < 0) of out,
+// laid out (n_comp, n_sets, n_w, n_t) complex.  A bubble nucleated at or after
+// t_end contributes zeros before the time range is checked (:681-692).
+void integrand_into(bgw_generalized_bulk_flow* h, const double* w_arr, size_t n_w,
+                    bool has_t_begin, double t_begin, double t_end, size_t n_t,
+                    const std::vector<int32_t>& ids, const Resolution* surface, double* out,
+                    std::array<int, 2> dst) {
+  const size_t n_sets = h->sc.n_sets;
+  const double tb = has_t_begin ? t_begin : 0.0;
+  bool live = false;
+  for (int32_t a : ids) live = live || h->bubbles.row(static_cast<size_t>(a))[0] < t_end;
+  std::vector<double> res_t(2 * 2 * n_sets * n_w * n_t, 0.0);  // (2, n_sets, n_t, n_w)
+  if (live) {
+    check_time_range(tb, t_end);
+    Request q;
+    q.res = surface ? surface : &need_resolution(h->res);
+    q.sc = &h->sc;
+    q.cutoff = Cutoff{};
+    q.n_w = n_w;
+    q.n_t = n_t;
+    q.identity = true;
+    q.k.push_back({0.0, 0.0, 1.0});
+    q.omega.assign(w_arr, w_arr + n_w);
+    q.t_begin = {tb};
+    q.t_end = {t_end};
+    q.bubbles_of_real = {ids};
+    q.skip_uncollided = false;
+    res_t = run_integrand(h->engine, q);
+  }
+  for (int pm = 0; pm < 2; ++pm) {
+    if (dst[pm] < 0) continue;
+    for (size_t s = 0; s < n_sets; ++s)
+      for (size_t w = 0; w < n_w; ++w)
+        for (size_t k = 0; k < n_t; ++k) {
+          const size_t src = (((pm * n_sets + s) * n_t + k) * n_w + w) * 2;
+          const size_t d = (((dst[pm] * n_sets + s) * n_w + w) * n_t + k) * 2;
+          out[d] = res_t[src];
+          out[d + 1] = res_t[src + 1];
+        }
+  }
+}
+
+Resolution tensor_surface(const Resolution& R, int pair) {
+  Resolution S = R;
+  const size_t n_c = R.n_c, n_phi = R.n_phi;
+  const double dphi = 2.0 * kPi / static_cast<double>(n_phi);
+  const double dcos = 2.0 / static_cast<double>(n_c - 1);
+  for (size_t j = 0; j < n_phi; ++j) {
+    const double w = (j == 0 || j == n_phi - 1) ? 0.5 : 1.0;
+    const double cp = R.cos_p[j], sp = R.sin_p[j];
+    double a = 0.0, b = 0.0;
+    switch (pair) {
+      case 0: a = cp * cp; b = cp * sp; break;
+      case 1: a = cp; b = sp; break;
+      case 2: a = sp * sp; break;
+      default: a = 1.0; break;
+    }
+    S.gcos[j] = a * w;
+    S.gsin[j] = b * w;
+  }
+  for (size_t i = 0; i < n_c; ++i) {
+    const double c = R.cos_t[i];
+    const double st = std::sqrt(std::fmax(1.0 - c * c, 0.0));
+    const double th = pair == 1 ? st * c : (pair == 3 ? c * c : st * st);
+    const double wi = (i == 0 || i == n_c - 1) ? 0.5 : 1.0;
+    S.ring_fac[i] = dcos * wi * th * dphi;
+  }
+  return S;
 }
 
 }  // namespace
@@ -1409,6 +1490,48 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integral_ex(
                                  kernel_mode(precision), out, out_len);
 }
 
+BgwStatus bgw_generalized_bulk_flow_compute_c_tensor(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
+    size_t n_selected_bubbles, BgwPrecision precision, BgwComplex64* out, size_t out_len) {
+  if (!valid_precision(precision)) {
+    set_last_error("precision is not a BgwPrecision");
+    return BgwStatus_InvalidArgument;
+  }
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    slice_from_raw(w_arr, n_w, "w_arr");
+    const size_t n_sets = h->sc.n_sets;
+    const size_t expected = 6 * n_sets * n_w;
+    if (out_len != expected)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(expected));
+    slice_from_raw(out, out_len, "out");
+    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+    bool empty = false;
+    const std::vector<int32_t> ids = selected_ids(h, selected_bubbles, n_selected_bubbles, &empty);
+    std::vector<double> res(2 * expected, 0.0);
+    if (!ids.empty()) {
+      check_time_range(has_t_begin ? t_begin : 0.0, t_end);
+      const Resolution& R = need_resolution(h->res);
+      const BgwTimeCutoffConfig unit = {BgwTimeCutoffKind_Unit, 0.0, NAN, NAN};
+      // launch -> (component of C+, component of Cx), -1: unused
+      const int comp[4][2] = {{0, 1}, {2, 4}, {3, -1}, {5, -1}};
+      const size_t blk = 2 * n_sets * n_w;  // doubles of one component
+      for (int pair = 0; pair < 4; ++pair) {
+        const Resolution S = tensor_surface(R, pair);
+        const std::vector<double> c = compute_c(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
+                                                selected_bubbles, n_selected_bubbles, unit,
+                                                kernel_mode(precision), &S);
+        for (int pm = 0; pm < 2; ++pm)
+          if (comp[pair][pm] >= 0)
+            std::memcpy(&res[comp[pair][pm] * blk], &c[pm * blk], blk * sizeof(double));
+      }
+    }
+    std::memcpy(out, res.data(), res.size() * sizeof(double));
+  });
+}
+
 BgwStatus bgw_generalized_bulk_flow_first_colliding_bubble(bgw_generalized_bulk_flow* bulk_flow,
                                                            size_t a_idx, int32_t* out,
                                                            size_t out_len) {
@@ -1544,39 +1667,37 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integrand(
       throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
                       std::to_string(expect));
     slice_from_raw(out, out_len, "out");
-    std::vector<double> res_t(expect * 2, 0.0);  // (2, n_sets, n_t, n_w)
-    const double tb = has_t_begin ? t_begin : 0.0;
-    // a bubble nucleated at or after t_end contributes zeros before the time
-    // range is checked (:681-692)
-    bool live = false;
-    for (int32_t a : ids) live = live || h->bubbles.row(static_cast<size_t>(a))[0] < t_end;
-    if (live) {
-      check_time_range(tb, t_end);
-      const Resolution& res = need_resolution(h->res);
-      Request q;
-      q.res = &res;
-      q.sc = &h->sc;
-      q.cutoff = Cutoff{};
-      q.n_w = n_w;
-      q.n_t = n_t;
-      q.identity = true;
-      q.k.push_back({0.0, 0.0, 1.0});
-      q.omega.assign(w_arr, w_arr + n_w);
-      q.t_begin = {tb};
-      q.t_end = {t_end};
-      q.bubbles_of_real = {ids};
-      q.skip_uncollided = false;
-      res_t = run_integrand(h->engine, q);
-    }
+    integrand_into(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t, ids, nullptr,
+                   reinterpret_cast<double*>(out), {0, 1});
+  });
+}
+
+BgwStatus bgw_generalized_bulk_flow_compute_c_tensor_integrand(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
+    size_t n_selected_bubbles, BgwComplex64* out, size_t out_len) {
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    slice_from_raw(w_arr, n_w, "w_arr");
+    const size_t n_sets = h->sc.n_sets;
+    // compute_c_tensor_integrand (new_generalized_bulk_flow.rs:993-1065)
+    if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+    bool empty = false;
+    std::vector<int32_t> ids = selected_ids(h, selected_bubbles, n_selected_bubbles, &empty);
+    const size_t expect = 6 * n_sets * n_w * n_t;
+    if (out_len != expect)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(expect));
+    slice_from_raw(out, out_len, "out");
     double* o = reinterpret_cast<double*>(out);
-    for (size_t ps = 0; ps < 2 * n_sets; ++ps)
-      for (size_t w = 0; w < n_w; ++w)
-        for (size_t k = 0; k < n_t; ++k) {
-          const size_t src = ((ps * n_t + k) * n_w + w) * 2;
-          const size_t dst = ((ps * n_w + w) * n_t + k) * 2;
-          o[dst] = res_t[src];
-          o[dst + 1] = res_t[src + 1];
-        }
+    std::fill(o, o + 2 * expect, 0.0);
+    const Resolution& R = need_resolution(h->res);
+    const int comp[4][2] = {{0, 1}, {2, 4}, {3, -1}, {5, -1}};
+    for (int pair = 0; pair < 4; ++pair) {
+      const Resolution S = tensor_surface(R, pair);
+      integrand_into(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t, ids, &S, o,
+                     {comp[pair][0], comp[pair][1]});
+    }
   });
 }
 

@@ -39,7 +39,7 @@ __all__ = [
     "Isometry3", "EmptyLattice", "ParallelepipedLattice", "CartesianLattice", "SphericalLattice",
     "LatticeBubbles", "ExponentialTimeCutoff", "SpontaneousNucleation", "FixedRateNucleation",
     "FixedRateNucleationResult", "nucleate_fixed_rate_batch", "GeneralizedBulkFlow",
-    "fibonacci_directions",
+    "NewGeneralizedBulkFlow", "fibonacci_directions",
 ]
 
 _PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED, "fp64_blocked": _lib.FP64_BLOCKED}
@@ -1227,3 +1227,60 @@ class GeneralizedBulkFlow:
         t = _lib.BgwTimings()
         self._call(_lib.lib().bgw_generalized_bulk_flow_last_timings, ctypes.byref(t))
         return {f: getattr(t, f) for f, _ in t._fields_}
+
+
+class NewGeneralizedBulkFlow(GeneralizedBulkFlow):
+    """The reference's experimental tensor form (new_generalized_bulk_flow.rs,
+    py_new_generalized_bulk_flow.rs): C_ij of the bubbles in the reference frame,
+    complex128 (6, n_sets, n_w) in the order xx, xy, xz, yy, yz, zz, without a time
+    cutoff.  The projections of the plain form are C+ = (C_xx - C_yy) / 2 and
+    Cx = C_xy.  Everything else (resolution, scaling parameters, masks) is that of
+    ``GeneralizedBulkFlow``."""
+
+    def _selection(self, selected_bubbles):
+        if selected_bubbles is None:
+            return None
+        sel_in = np.asarray(selected_bubbles)
+        if sel_in.ndim != 1:
+            raise ValueError("selected_bubbles must be a contiguous 1-D array in C order")
+        for a in sel_in.tolist():
+            if a < 0 or a >= len(self._interior):
+                raise IndexError(f"Index {a} out of bounds (max: {len(self._interior)})")
+        return np.ascontiguousarray(sel_in.astype(np.uintp))
+
+    def compute_c_tensor(self, w_arr, *, t_begin=None, t_end, n_t, selected_bubbles=None,
+                         precision="fp64"):
+        """C_ij, complex128 (6, n_sets, n_w) (new_generalized_bulk_flow.rs:1205-1275)."""
+        w = np.ascontiguousarray(np.asarray(w_arr, dtype=np.float64).reshape(-1))
+        sel = self._selection(selected_bubbles)
+        self._push_params()
+        out = np.zeros((6, self._coef.shape[0], len(w)), dtype=np.complex128)
+        self._call(_lib.lib().bgw_generalized_bulk_flow_compute_c_tensor, _lib.dptr(w), len(w),
+                   t_begin is not None, 0.0 if t_begin is None else float(t_begin), float(t_end),
+                   int(n_t), _lib.sptr(sel) if sel is not None else None,
+                   0 if sel is None else len(sel), _precision(precision),
+                   out.ctypes.data_as(_lib._dp), out.size)
+        return out
+
+    def compute_c_tensor_fixed_bubble(self, a_idx, w_arr, t_begin, t_end, n_t):
+        """One bubble's C_ij (new_generalized_bulk_flow.rs:1067-1203)."""
+        self._check_index(a_idx)
+        return self.compute_c_tensor(w_arr, t_begin=t_begin, t_end=t_end, n_t=n_t,
+                                     selected_bubbles=np.array([a_idx]))
+
+    def compute_c_tensor_integrand(self, w_arr, *, t_begin=None, t_end, n_t, selected_bubbles=None):
+        """Time-resolved integrand of C_ij, complex128 (6, n_sets, n_w, n_t) (:993-1065)."""
+        w = np.ascontiguousarray(np.asarray(w_arr, dtype=np.float64).reshape(-1))
+        sel = self._selection(selected_bubbles)
+        self._push_params()
+        out = np.zeros((6, self._coef.shape[0], len(w), int(n_t)), dtype=np.complex128)
+        self._call(_lib.lib().bgw_generalized_bulk_flow_compute_c_tensor_integrand, _lib.dptr(w),
+                   len(w), t_begin is not None, 0.0 if t_begin is None else float(t_begin),
+                   float(t_end), int(n_t), _lib.sptr(sel) if sel is not None else None,
+                   0 if sel is None else len(sel), out.ctypes.data_as(_lib._dp), out.size)
+        return out
+
+    def compute_c_tensor_integrand_fixed_bubble(self, a_idx, w_arr, t_begin, t_end, n_t):
+        self._check_index(a_idx)
+        return self.compute_c_tensor_integrand(w_arr, t_begin=t_begin, t_end=t_end, n_t=n_t,
+                                               selected_bubbles=np.array([a_idx]))

@@ -305,6 +305,26 @@ BgwStatus bgw_generalized_bulk_flow_set_tiles(bgw_generalized_bulk_flow *bulk_fl
 BgwStatus bgw_generalized_bulk_flow_last_timings(bgw_generalized_bulk_flow *bulk_flow,
                                                  BgwTimings *out);
 
+/* Reference: NewGeneralizedBulkFlow::compute_c_tensor (rust/bubble-gw/src/many_bubbles/
+ * new_generalized_bulk_flow.rs:1205-1275, the experimental tensor form): C_ij of the
+ * bubbles in the reference frame, out (6, n_sets, n_w) in the order xx, xy, xz, yy, yz,
+ * zz (out_len = 6 n_sets n_w); no time cutoff.  C+ = (C_xx - C_yy) / 2 and Cx = C_xy. */
+BgwStatus bgw_generalized_bulk_flow_compute_c_tensor(bgw_generalized_bulk_flow *bulk_flow,
+                                                     const double *w_arr, size_t n_w,
+                                                     bool has_t_begin, double t_begin,
+                                                     double t_end, size_t n_t,
+                                                     const size_t *selected_bubbles,
+                                                     size_t n_selected_bubbles,
+                                                     BgwPrecision precision, BgwComplex64 *out,
+                                                     size_t out_len);
+
+/* Reference: NewGeneralizedBulkFlow::compute_c_tensor_integrand (new_generalized_bulk_flow.rs:
+ * 993-1065): the per-step integrand of compute_c_tensor, out (6, n_sets, n_w, n_t). */
+BgwStatus bgw_generalized_bulk_flow_compute_c_tensor_integrand(
+    bgw_generalized_bulk_flow *bulk_flow, const double *w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t *selected_bubbles,
+    size_t n_selected_bubbles, BgwComplex64 *out, size_t out_len);
+
 /* ---- additions: batched ensemble ------------------------------------------- */
 
 /* Realisation r owns rows [interior_offsets[r], interior_offsets[r+1]) of

@@ -58,6 +58,9 @@ def lib():
         L.oracle_c_integrand.argtypes = [
             _dp, _sz, _dp, _sz, _sz, _sz, _dp, _dp, _sz, _sz, _d,
             _dp, _sz, _i, _d, _d, _sz, _sp, _sz, _i, _i, _dp]
+        L.oracle_c_tensor.argtypes = [
+            _dp, _sz, _dp, _sz, _sz, _sz, _dp, _dp, _sz, _sz, _d, _dp, _sz, _i, _d, _d, _sz,
+            _sp, _sz, _i, _i, _dp]
         L.oracle_spectrum.argtypes = [
             _dp, _sz, _dp, _sz, _dp, _sz, _sz, _sz, _dp, _dp, _sz, _sz, _d,
             _dp, _sz, _i, _d, _d, _sz, _i, _d, _d, _d, _i, _dp, _dp]
@@ -225,3 +228,24 @@ def spectrum(interior, exterior, k_hat, n_c, n_phi, coefficients_sets, powers_se
         0.0 if t_begin is None else t_begin, t_end, n_t, kind, smax, rc, r0, n_threads,
         _ptr(S), None if C is None else C.ctypes.data_as(_dp)))
     return (S, C) if return_c else S
+
+
+def compute_c_tensor(interior, exterior, n_c, n_phi, coefficients_sets, powers_sets, w_arr, *,
+                     t_end, n_t, t_begin=None, selected_bubbles=None, damping=None, n_threads=0):
+    """NewGeneralizedBulkFlow.compute_c_tensor -> complex128 (6, n_sets, n_w), components
+    xx, xy, xz, yy, yz, zz (new_generalized_bulk_flow.rs:1205-1275)."""
+    interior, exterior = _bubbles(interior, exterior)
+    coef, pows = _params(coefficients_sets, powers_sets)
+    w = _f64(w_arr)
+    n_s, n_terms = coef.shape
+    out = np.zeros((6, n_s, len(w)), dtype=np.complex128)
+    sel = None
+    if selected_bubbles is not None:
+        sel = np.ascontiguousarray(np.asarray(selected_bubbles, dtype=np.uint64)).astype(np.uintp)
+    _check(lib().oracle_c_tensor(
+        _ptr(interior), len(interior), _ptr(exterior), len(exterior), n_c, n_phi, _ptr(coef),
+        _ptr(pows), n_s, n_terms, float("nan") if damping is None else damping, _ptr(w), len(w),
+        int(t_begin is not None), 0.0 if t_begin is None else t_begin, t_end, n_t,
+        None if sel is None else sel.ctypes.data_as(_sp), 0 if sel is None else len(sel),
+        int(sel is not None), n_threads, out.ctypes.data_as(_dp)))
+    return out

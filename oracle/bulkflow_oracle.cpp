@@ -683,6 +683,137 @@ bool c_integral(Common& c, const double* w_arr, size_t n_w, double t_begin, doub
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// Tensor form of the experimental NewGeneralizedBulkFlow
+// (new_generalized_bulk_flow.rs); components in the order xx, xy, xz, yy, yz, zz.
+// ---------------------------------------------------------------------------
+
+// compute_b_tensor (:592-727) for one cos-theta row; b is (6, n_sets).
+void b_tensor(const Flow& F, size_t i, const int32_t* status_row, const double* dtab_row,
+              double delta_ta, double* b) {
+  const size_t n_phi = F.n_phi, ns = F.n_sets;
+  const double cos_t = F.cos_t[i];
+  const double sin_t = std::sqrt(std::fmax(1.0 - cos_t * cos_t, 0.0));
+  const double sin_sq = sin_t * sin_t;
+  const double cos_sq = cos_t * cos_t;
+  const double sin_cos = sin_t * cos_t;
+  const double dphi = 2.0 * kPi / static_cast<double>(n_phi);
+  std::vector<double> basis[6];
+  for (auto& v : basis) v.resize(n_phi);
+  for (size_t j = 0; j < n_phi; ++j) {
+    double sp, cp;
+    ::sincos(F.phi[j], &sp, &cp);  // p.cos(), p.sin() in one closure
+    basis[0][j] = (cp * cp) * sin_sq;
+    basis[1][j] = (cp * sp) * sin_sq;
+    basis[2][j] = cp * sin_cos;
+    basis[3][j] = (sp * sp) * sin_sq;
+    basis[4][j] = sp * sin_cos;
+    basis[5][j] = cos_sq;
+  }
+  std::vector<double> w(n_phi, 1.0), fac(n_phi), tmp(n_phi);
+  w[0] = 0.5;
+  w[n_phi - 1] = 0.5;
+  const double dta3 = delta_ta > 0.0 ? (delta_ta * delta_ta) * delta_ta : 0.0;
+  for (size_t s = 0; s < ns; ++s) {
+    for (size_t j = 0; j < n_phi; ++j) {
+      if (status_row[j] != kAlready) {
+        fac[j] = 1.0;
+        continue;
+      }
+      const double ratio = dtab_row[j] / delta_ta;
+      double f = 0.0;
+      for (size_t k = 0; k < F.n_terms; ++k)
+        f += F.coef[s * F.n_terms + k] * std::pow(ratio, F.pows[s * F.n_terms + k]);
+      if (F.has_damping) f *= std::exp((-delta_ta) * (1.0 - ratio) / F.damping);
+      fac[j] = f;
+    }
+    for (int c = 0; c < 6; ++c) {
+      for (size_t j = 0; j < n_phi; ++j) tmp[j] = basis[c][j] * fac[j] * w[j];
+      b[c * ns + s] = ndarray_sum(tmp.data(), n_phi) * dphi * dta3;
+    }
+  }
+}
+
+// compute_a_tensor (:729-830); a is (6, n_sets, n_w).
+void a_tensor(const Lattice& L, const Flow& F, size_t a_idx, const double* w_arr, size_t n_w,
+              double t, const std::vector<int64_t>& first, const std::vector<double>& dtab, Cx* a) {
+  const size_t n_c = F.n_c, n_phi = F.n_phi, ns = F.n_sets;
+  const double delta_ta = t - L.interior[a_idx * 4 + 0];
+  std::vector<int32_t> status(n_c * n_phi);
+  for (size_t ij = 0; ij < n_c * n_phi; ++ij)
+    status[ij] = collision_status(first[ij], dtab[ij], delta_ta);
+  for (size_t q = 0; q < 6 * ns * n_w; ++q) a[q] = {0.0, 0.0};
+  const double dcos = 2.0 / static_cast<double>(n_c - 1);
+  std::vector<double> b(6 * ns);
+  std::vector<Cx> ph(n_w);
+  for (size_t i = 0; i < n_c; ++i) {
+    b_tensor(F, i, &status[i * n_phi], &dtab[i * n_phi], delta_ta, b.data());
+    const double base_im = (-delta_ta) * F.cos_t[i];
+    const double weight = (i == 0 || i == n_c - 1) ? 0.5 : 1.0;
+    const double scalar = dcos * weight;
+    for (size_t q = 0; q < n_w; ++q) ph[q] = cscale(cexp_i(w_arr[q] * base_im), scalar);
+    for (int c = 0; c < 6; ++c)
+      for (size_t s = 0; s < ns; ++s) {
+        const Cx bv = {b[c * ns + s], 0.0};
+        for (size_t q = 0; q < n_w; ++q) {
+          const Cx u = cmul(bv, ph[q]);
+          Cx& dst = a[(c * ns + s) * n_w + q];
+          dst.re += u.re;
+          dst.im += u.im;
+        }
+      }
+  }
+}
+
+// compute_c_tensor_fixed_bubble (:1067-1203); out is (6, n_sets, n_w).  No
+// time cutoff; steps with t <= t_a are skipped; the rayon fold over t is
+// summed sequentially in index order.
+void c_tensor_fixed(const Lattice& L, const Flow& F, size_t a_idx, const double* w_arr, size_t n_w,
+                    double t_begin, double t_end, size_t n_t, Cx* out) {
+  const size_t nq = 6 * F.n_sets * n_w;
+  std::vector<double> t_arr(n_t);
+  for (size_t k = 0; k < n_t; ++k) t_arr[k] = linspace_at(t_begin, t_end, n_t, k);
+  const double dt = n_t > 1 ? t_arr[1] - t_arr[0] : 0.0;
+  const double z_a = L.interior[a_idx * 4 + 3];
+  const double t_nuc = L.interior[a_idx * 4 + 0];
+  std::vector<int64_t> first;
+  first_colliding(L, F, a_idx, first);
+  std::vector<double> dtab;
+  delta_tab(L, F, a_idx, first, dtab);
+  std::vector<Cx> contrib(n_t * nq, Cx{0.0, 0.0});
+  std::vector<char> active(n_t, 0);
+  const long nt = static_cast<long>(n_t);
+#pragma omp parallel num_threads(F.n_threads)
+  {
+    std::vector<Cx> at(nq);
+#pragma omp for schedule(dynamic, 1)
+    for (long k = 0; k < nt; ++k) {
+      const double t = t_arr[k];
+      if (t <= t_nuc) continue;
+      active[k] = 1;
+      a_tensor(L, F, a_idx, w_arr, n_w, t, first, dtab, at.data());
+      const double weight = (k == 0 || k == nt - 1) ? 0.5 : 1.0;
+      const Cx dtc = {dt * weight, 0.0};
+      Cx* o = &contrib[k * nq];
+      for (size_t cs = 0; cs < 6 * F.n_sets; ++cs)
+        for (size_t q = 0; q < n_w; ++q) {
+          const Cx factor = cmul(cexp_i(w_arr[q] * (t - z_a)), dtc);
+          o[cs * n_w + q] = cmul(at[cs * n_w + q], factor);
+        }
+    }
+  }
+  std::vector<Cx> acc(nq, Cx{0.0, 0.0});
+  for (size_t k = 0; k < n_t; ++k) {
+    if (!active[k]) continue;
+    for (size_t q = 0; q < nq; ++q) {
+      acc[q].re += contrib[k * nq + q].re;
+      acc[q].im += contrib[k * nq + q].im;
+    }
+  }
+  const Cx factor = {1.0 / (6.0 * kPi), 0.0};
+  for (size_t q = 0; q < nq; ++q) out[q] = cmul(acc[q], factor);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -793,6 +924,44 @@ int oracle_c_integral(const double* interior, size_t n_int, const double* exteri
                     reinterpret_cast<Cx*>(out))
              ? 0
              : 3;
+}
+
+// NewGeneralizedBulkFlow::compute_c_tensor (new_generalized_bulk_flow.rs:1205-1275).
+// out: (6, n_sets, n_w) complex, components xx, xy, xz, yy, yz, zz.
+int oracle_c_tensor(const double* interior, size_t n_int, const double* exterior, size_t n_ext,
+                    size_t n_c, size_t n_phi, const double* coef, const double* pows,
+                    size_t n_sets, size_t n_terms, double damping, const double* w_arr,
+                    size_t n_w, int has_t_begin, double t_begin, double t_end, size_t n_t,
+                    const size_t* selected, size_t n_selected, int has_selected, int n_threads,
+                    double* out) {
+  Common c;
+  if (!setup(c, interior, n_int, exterior, n_ext, n_c, n_phi, coef, pows, n_sets, n_terms, damping,
+             n_threads))
+    return 3;
+  if (n_t < 2) {
+    g_last_error = "Invalid resolution: n_t must be >= 2 for integration";
+    return 3;
+  }
+  std::vector<size_t> ids;
+  if (!validate_selected(selected, n_selected, has_selected != 0, n_int, ids)) return 3;
+  const size_t nq = 6 * n_sets * n_w;
+  Cx* res = reinterpret_cast<Cx*>(out);
+  for (size_t q = 0; q < nq; ++q) res[q] = {0.0, 0.0};
+  const double tb = has_t_begin ? t_begin : 0.0;
+  if (!ids.empty() && tb > t_end) {
+    g_last_error = "Invalid time range: t_begin=" + std::to_string(tb) +
+                   " > t_end=" + std::to_string(t_end);
+    return 3;
+  }
+  std::vector<Cx> one(nq);
+  for (size_t a : ids) {
+    c_tensor_fixed(c.L, c.F, a, w_arr, n_w, tb, t_end, n_t, one.data());
+    for (size_t q = 0; q < nq; ++q) {
+      res[q].re += one[q].re;
+      res[q].im += one[q].im;
+    }
+  }
+  return 0;
 }
 
 // compute_c_integrand (:875-920).  out: (2, n_sets, n_w, n_t) complex.

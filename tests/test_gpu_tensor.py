@@ -1,0 +1,61 @@
+"""The tensor form (NewGeneralizedBulkFlow) on the GPU against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from gw_fopt_amd import configs
+from gw_fopt_amd import many_bubbles as mb
+from gw_fopt_amd import nucleation
+
+pytestmark = pytest.mark.gpu
+
+
+def _flow(inter, ext, n_c, n_phi, coef, pows):
+    bf = mb.NewGeneralizedBulkFlow(mb.LatticeBubbles(mb.EmptyLattice(), inter, ext))
+    bf.set_resolution(n_c, n_phi)
+    bf.set_gradient_scaling_params(coef, pows)
+    return bf
+
+
+def _rel_per_component(a, b):
+    return max(np.max(np.abs(a[c] - b[c])) / np.max(np.abs(b[c])) for c in range(6))
+
+
+def test_tensor_plain_8_matches_oracle():
+    wl = configs.plain_8()
+    bf = _flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, [[1.0], [0.5]], [[5.0], [3.5]])
+    w = np.geomspace(0.5, 80.0, 16)
+    T = bf.compute_c_tensor(w, t_end=wl.t_end, n_t=wl.n_t)
+    To = oracle.compute_c_tensor(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix,
+                                 [[1.0], [0.5]], [[5.0], [3.5]], w, t_end=wl.t_end, n_t=wl.n_t)
+    assert T.shape == (6, 2, 16)
+    assert _rel_per_component(T, To) <= 1e-10
+    Tb = bf.compute_c_tensor(w, t_end=wl.t_end, n_t=wl.n_t, precision="fp64_blocked")
+    assert _rel_per_component(Tb, To) <= 1e-9
+    # the plain form's projections from the same handle
+    C = bf.compute_c_integral(w, t_end=wl.t_end, n_t=wl.n_t)
+    assert np.max(np.abs((T[0] - T[3]) / 2 - C[0])) <= 1e-10 * np.max(np.abs(C[0]))
+
+
+def test_tensor_selected_and_integrand():
+    inter = nucleation.exponential_nucleation(12.0, n_max=6, seed=11)
+    ext = nucleation.face_images(inter, 1.0)
+    bf = _flow(inter, ext, 12, 24, [[1.0]], [[5.0]])
+    w = np.array([0.6, 3.0, 17.0])
+    sel = [0, 3, 5]
+    T = bf.compute_c_tensor(w, t_end=1.5, n_t=70, selected_bubbles=sel)
+    To = oracle.compute_c_tensor(inter, ext, 12, 24, [[1.0]], [[5.0]], w, t_end=1.5, n_t=70,
+                                 selected_bubbles=sel)
+    assert _rel_per_component(T, To) <= 1e-10
+    one = bf.compute_c_tensor_fixed_bubble(3, w, None, 1.5, 70)
+    one_o = oracle.compute_c_tensor(inter, ext, 12, 24, [[1.0]], [[5.0]], w, t_end=1.5, n_t=70,
+                                    selected_bubbles=[3])
+    assert _rel_per_component(one, one_o) <= 1e-10
+    I = bf.compute_c_tensor_integrand(w, t_end=1.5, n_t=70, selected_bubbles=sel)
+    assert I.shape == (6, 1, 3, 70)
+    assert _rel_per_component(I.sum(axis=-1), To) <= 1e-10
+    assert not np.any(bf.compute_c_tensor(w, t_end=1.5, n_t=70, selected_bubbles=[]))
+    with pytest.raises(IndexError):
+        bf.compute_c_tensor(w, t_end=1.5, n_t=70, selected_bubbles=[7])
+    with pytest.raises(ValueError, match="n_t must be >= 2"):
+        bf.compute_c_tensor(w, t_end=1.5, n_t=1)
