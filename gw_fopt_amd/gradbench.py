@@ -70,7 +70,7 @@ class GradProblem:
         return t.detach().numpy().copy(), hist
 
 
-def measure(steps: int = 5, fit_steps: int = 200) -> dict:
+def measure(steps: int = 20, fit_steps: int = 200) -> dict:
     """Gradients per second (forward + adjoint), the FD check and the fit.
 
     The timed gradients and the fit use the blocked FP64 forward (within
@@ -82,18 +82,21 @@ def measure(steps: int = 5, fit_steps: int = 200) -> dict:
     fd = P64.central_fd(theta, 1e-6)
     rel = float(np.max(np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)))
     P = GradProblem(P64.wl, precision="fp64_blocked")
-    P.loss_and_grad(theta)  # warm-up
-    t0 = time.perf_counter()
-    dev_ms = []
+    for _ in range(3):  # warm-up
+        P.loss_and_grad(theta)
+    dev_ms, walls = [], []
     for _ in range(steps):
+        t0 = time.perf_counter()
         _, gb = P.loss_and_grad(theta)
+        walls.append(time.perf_counter() - t0)
         dev_ms.append(P.bf.last_timings()["total_ms"])
-    wall = (time.perf_counter() - t0) / steps
+    wall = float(np.median(walls))
     t1 = time.perf_counter()
     th, hist = P.fit(steps=fit_steps)
     fit_s = time.perf_counter() - t1
     return {"grad_per_s": 1.0 / wall, "wall_ms_per_grad": wall * 1e3,
-            "adjoint_device_ms": float(np.mean(dev_ms)),
+            "wall_ms_per_grad_mean": float(np.mean(walls)) * 1e3,
+            "adjoint_device_ms": float(np.median(dev_ms)),
             "grad": g.tolist(), "fd_h1e-6": fd.tolist(), "fd_max_rel_err": rel,
             "grad_blocked_forward": gb.tolist(),
             "fit": {"theta0": [1.5, 1.3], "steps": fit_steps, "theta": th.tolist(),
