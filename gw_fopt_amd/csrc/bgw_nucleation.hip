@@ -249,7 +249,8 @@ __device__ inline FalseVacuum block_reduce(FalseVacuum v, ReduceSlots& s, int pa
   for (int off = 32; off >= 1; off >>= 1) {
     v.count += __shfl_xor(v.count, off, 64);
     v.first = min(v.first, static_cast<uint32_t>(__shfl_xor(v.first, off, 64)));
-    v.next = fmin(v.next, __shfl_xor(v.next, off, 64));
+    const double o = __shfl_xor(v.next, off, 64);
+    v.next = o < v.next ? o : v.next;
   }
   if (lane == 0) {
     s.count[parity][wave] = v.count;
@@ -262,7 +263,7 @@ __device__ inline FalseVacuum block_reduce(FalseVacuum v, ReduceSlots& s, int pa
   for (int w = 0; w < kNucWaves; ++w) {
     r.count += s.count[parity][w];
     r.first = min(r.first, s.first[parity][w]);
-    r.next = fmin(r.next, s.next[parity][w]);
+    r.next = s.next[parity][w] < r.next ? s.next[parity][w] : r.next;
   }
   return r;
 }
@@ -324,7 +325,7 @@ struct ReachTimes {
         const double x = f[k];
         const bool live = x > t;
         v.count += live ? 1u : 0u;
-        v.next = live ? fmin(v.next, x) : v.next;
+        v.next = (live && x < v.next) ? x : v.next;
         v.first = live ? uint32_t(threadIdx.x * K + k) : v.first;
       }
     } else {
@@ -332,7 +333,7 @@ struct ReachTimes {
         const double x = g[i];
         if (x > t) {
           v.count += 1u;
-          v.next = fmin(v.next, x);
+          v.next = x < v.next ? x : v.next;
           v.first = min(v.first, i);
         }
       }
@@ -383,6 +384,7 @@ void fixed_rate_kernel(FixedRateArgs a) {
   __shared__ ReduceSlots slots;
   __shared__ uint32_t scan_slots[kNucWaves];
   __shared__ double s_src[1 + kMaxImages][4];  // the new bubble and its images
+  __shared__ uint8_t s_accept[4 * kNucThreads];  // time stepping: draw <= d_p0 per iteration
   const int r = blockIdx.x;
   const NucReal& R = a.reals[r];
   const DevLattice& L = R.lat;
@@ -477,26 +479,40 @@ void fixed_rate_kernel(FixedRateArgs a) {
 
   if (a.method == BgwFixedRateMethod_TimeStepping) {
     double t = R.t0;
-    U4 draws{{0, 0, 0, 0}};
+    // The acceptance draws do not depend on the state: the workgroup evaluates
+    // them 4 * kNucThreads iterations at a time (one Philox block per lane)
+    // into LDS, off the sequential chain.
+    uint64_t draw_base = 0;
+    // the false-vacuum volume and V / V_fv change only with the count
+    uint32_t count_seen = 0xFFFFFFFFu;
+    double volume_remaining = 0.0, volume_ratio = 0.0;
     for (uint64_t it = 0; it < a.max_iter; ++it) {
       refresh(t);
-      const double fraction = static_cast<double>(fv.count) / static_cast<double>(np);
-      const double volume_remaining = fmax(fraction * volume_lattice, 0.0);
+      if (fv.count != count_seen) {
+        count_seen = fv.count;
+        const double fraction = static_cast<double>(fv.count) / static_cast<double>(np);
+        volume_remaining = fmax(fraction * volume_lattice, 0.0);
+        volume_ratio = volume_lattice / volume_remaining;
+      }
       if (volume_remaining < volume_cutoff) break;
       const double exponent = R.beta * (t - R.t0);
       const double gamma_t = R.gamma0 * exp(exponent);
-      if ((it & 3) == 0) draws = philox(U4{{it >> 2, kStreamSteps, 0, 0}}, R.seed, R.stream);
-      const uint64_t w = (it & 3) == 0 ? draws.v[0] : (it & 3) == 1 ? draws.v[1]
-                       : (it & 3) == 2 ? draws.v[2] : draws.v[3];
-      const double x = uniform(w);
-      if (x <= a.d_p0 && fv.count > 0) {
+      if (it == 0 || it - draw_base >= 4 * kNucThreads) {
+        draw_base = it;  // a multiple of 4 * kNucThreads
+        __syncthreads();  // every lane is done with the previous span
+        const U4 x = philox(U4{{(it >> 2) + threadIdx.x, kStreamSteps, 0, 0}}, R.seed, R.stream);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_accept[4 * threadIdx.x + q] = uniform(x.v[q]) <= a.d_p0;
+        __syncthreads();
+      }
+      if (s_accept[it - draw_base] && fv.count > 0) {
         if (nb >= cap) {
           status = 2;
           break;
         }
         nucleate(t, volume_remaining, t);
       }
-      const double dt = (a.d_p0 / gamma_t) * (volume_lattice / volume_remaining);
+      const double dt = (a.d_p0 / gamma_t) * volume_ratio;
       t += dt;
     }
   } else {
