@@ -388,6 +388,9 @@ void fixed_rate_kernel(FixedRateArgs a) {
   const int r = blockIdx.x;
   const NucReal& R = a.reals[r];
   const DevLattice& L = R.lat;
+  // scalars of the sequential loop in registers (not re-read from memory every step)
+  const double beta = R.beta, gamma0 = R.gamma0, t_start = R.t0;
+  const uint64_t seed = R.seed, stream = R.stream;
   const uint32_t np = a.n_points;
   double* px = a.px + size_t(r) * np;
   double* py = a.py + size_t(r) * np;
@@ -400,7 +403,7 @@ void fixed_rate_kernel(FixedRateArgs a) {
     const uint64_t att = base + threadIdx.x;
     double p[3] = {0, 0, 0};
     bool ok = false;
-    if (att < max_att) ok = sample_point(L, philox(U4{{att, kStreamPoints, 0, 0}}, R.seed, R.stream), p);
+    if (att < max_att) ok = sample_point(L, philox(U4{{att, kStreamPoints, 0, 0}}, seed, stream), p);
     const Scan s = block_scan(ok, scan_slots);
     const uint64_t idx = got + s.before;
     if (ok && idx < np) {
@@ -478,7 +481,7 @@ void fixed_rate_kernel(FixedRateArgs a) {
   };
 
   if (a.method == BgwFixedRateMethod_TimeStepping) {
-    double t = R.t0;
+    double t = t_start;
     // The acceptance draws do not depend on the state: the workgroup evaluates
     // them 4 * kNucThreads iterations at a time (one Philox block per lane)
     // into LDS, off the sequential chain.
@@ -495,12 +498,12 @@ void fixed_rate_kernel(FixedRateArgs a) {
         volume_ratio = volume_lattice / volume_remaining;
       }
       if (volume_remaining < volume_cutoff) break;
-      const double exponent = R.beta * (t - R.t0);
-      const double gamma_t = R.gamma0 * exp(exponent);
+      const double exponent = beta * (t - t_start);
+      const double gamma_t = gamma0 * exp(exponent);
       if (it == 0 || it - draw_base >= 4 * kNucThreads) {
         draw_base = it;  // a multiple of 4 * kNucThreads
         __syncthreads();  // every lane is done with the previous span
-        const U4 x = philox(U4{{(it >> 2) + threadIdx.x, kStreamSteps, 0, 0}}, R.seed, R.stream);
+        const U4 x = philox(U4{{(it >> 2) + threadIdx.x, kStreamSteps, 0, 0}}, seed, stream);
 #pragma unroll
         for (int q = 0; q < 4; ++q) s_accept[4 * threadIdx.x + q] = uniform(x.v[q]) <= a.d_p0;
         __syncthreads();
