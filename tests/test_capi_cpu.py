@@ -266,3 +266,37 @@ def test_ensemble_validation():
     assert _lib.last_error() == "tiles.persistent_blocks must be <= 65536"
     L.bgw_ensemble_free(h)
     L.bgw_ensemble_free(None)
+
+
+def test_tensor_validation_and_zero_cases():
+    """compute_c_tensor / compute_c_tensor_integrand (new_generalized_bulk_flow.rs:993-1275):
+    n_t first, out_len, an empty selection and late bubbles give zeros without a device."""
+    L = _lib.lib()
+    inter = np.array([[0.0, 0.0, 10.0, 0.0], [2.0, 0.0, 0.0, 0.0]])
+    h = _handle(inter)
+    L.bgw_generalized_bulk_flow_set_resolution(h, 4, 5, True)
+    w = np.array([0.5, 1.0])
+    out = np.full(6 * 1 * 2 * 2, 7.0)
+    st = L.bgw_generalized_bulk_flow_compute_c_tensor(h, _lib.dptr(w), 2, True, 0.0, 1.0, 1, None, 0,
+                                                      _lib.FP64, _lib.dptr(out), 12)
+    assert st == _lib.CALCULATION_ERROR
+    assert _lib.last_error() == "Invalid resolution: n_t must be >= 2 for integration"
+    st = L.bgw_generalized_bulk_flow_compute_c_tensor(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6, None, 0,
+                                                      _lib.FP64, _lib.dptr(out), 11)
+    assert "out_len=11 does not match expected length 12" in _lib.last_error()
+    empty = np.zeros(0, dtype=np.uintp)
+    st = L.bgw_generalized_bulk_flow_compute_c_tensor(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6,
+                                                      _lib.sptr(empty), 0, _lib.FP64, _lib.dptr(out), 12)
+    assert st == _lib.OK and np.all(out == 0.0)
+    bad = np.array([5], dtype=np.uintp)
+    st = L.bgw_generalized_bulk_flow_compute_c_tensor(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6,
+                                                      _lib.sptr(bad), 1, _lib.FP64, _lib.dptr(out), 12)
+    assert st == _lib.CALCULATION_ERROR and "out of bounds" in _lib.last_error()
+    assert L.bgw_generalized_bulk_flow_compute_c_tensor(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6, None, 0,
+                                                        9, _lib.dptr(out), 12) == _lib.INVALID_ARGUMENT
+    outi = np.full(6 * 2 * 2 * 6, 7.0)
+    sel = np.array([1], dtype=np.uintp)  # nucleated at t = 2 > t_end: zeros, no device
+    st = L.bgw_generalized_bulk_flow_compute_c_tensor_integrand(h, _lib.dptr(w), 2, True, 0.0, 1.0, 6,
+                                                                _lib.sptr(sel), 1, _lib.dptr(outi), 72)
+    assert st == _lib.OK and np.all(outi == 0.0), _lib.last_error()
+    L.bgw_generalized_bulk_flow_free(h)
