@@ -44,6 +44,23 @@ class BubbleEnsemble:
         self._n_sets = 1
         self.set_gradient_scaling_params([[1.0]], [[3.0]])
 
+    @classmethod
+    def from_nucleation(cls, interior, exterior, counts) -> "BubbleEnsemble":
+        """An ensemble from a device nucleation batch (``nucleate_fixed_rate_batch`` or
+        ``FixedRateNucleation.nucleate_many``): realisation r is rows [0, counts[r]) of
+        interior[r] and the matching counts[r] * n_img rows of exterior[r]."""
+        interior = np.asarray(interior, dtype=np.float64)
+        exterior = np.asarray(exterior, dtype=np.float64)
+        counts = np.asarray(counts, dtype=np.int64).reshape(-1)
+        if interior.ndim != 3 or interior.shape[0] != len(counts) or interior.shape[2] != 4:
+            raise ValueError("interior must be (n_realisations, max_bubbles, 4)")
+        cap = interior.shape[1]
+        n_img = exterior.shape[1] // cap if cap else 0
+        if np.any(counts < 0) or np.any(counts > cap):
+            raise ValueError("counts must lie in [0, max_bubbles]")
+        return cls([interior[r, :n] for r, n in enumerate(counts)],
+                   [exterior[r, :n * n_img] for r, n in enumerate(counts)])
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -155,3 +155,33 @@ def test_nucleated_bubbles_feed_the_spectrum():
     got = bf.compute_c_integral(w, t_end=1.0, n_t=40)
     want = oracle.compute_c_integral(interior, exterior, 8, 16, [[1.0]], [[5.0]], w, t_end=1.0, n_t=40)
     np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+
+
+def test_device_nucleation_feeds_the_ensemble():
+    """Realisations nucleated on the device go straight into BubbleEnsemble; each
+    realisation's spectrum equals GeneralizedBulkFlow on its bubbles bit for bit, and
+    the oracle within 1e-10."""
+    from gw_fopt_amd.ensemble import BubbleEnsemble
+    lat = mb.CartesianLattice((0.0, 0.0, 0.0), np.eye(3))
+    fr = mb.FixedRateNucleation(beta=7.0, gamma0=7.0**4 * 0.01, t0=0.0, d_p0=0.05, seed=21,
+                                n_points=600, max_time_iterations=400)
+    interior, exterior, _, _, counts = fr.nucleate_many(
+        lat, 6, method="fixed_probability_time_stepping", d_p0=0.05, max_bubbles=64)
+    assert counts.min() >= 2
+    E = BubbleEnsemble.from_nucleation(interior, exterior, counts)
+    E.set_resolution(8, 16)
+    E.set_gradient_scaling_params([[1.0]], [[5.0]])
+    w = np.geomspace(0.5, 20.0, 6)
+    k = mb.fibonacci_directions(3)
+    t_end = np.array([interior[r, counts[r] - 1, 0] + 0.5 for r in range(6)])
+    out = E.compute_spectrum(w, k, t_end=t_end, n_t=40, precision="fp64")
+    for r in (0, 5):
+        n = int(counts[r])
+        inter, ext = interior[r, :n], exterior[r, :6 * n]
+        bf = mb.GeneralizedBulkFlow(mb.LatticeBubbles(mb.EmptyLattice(), inter, ext))
+        bf.set_resolution(8, 16)
+        bf.set_gradient_scaling_params([[1.0]], [[5.0]])
+        S, _ = bf.compute_spectrum(w, k, t_end=t_end[r], n_t=40)
+        assert out["S"][r].tobytes() == S.tobytes()
+        So = oracle.spectrum(inter, ext, k, 8, 16, [[1.0]], [[5.0]], w, t_end=t_end[r], n_t=40)
+        assert np.max(np.abs(S - So) / np.abs(So)) <= 1e-10
