@@ -38,13 +38,20 @@ namespace bgw {
 // ---------------------------------------------------------------------------
 // Stage 1: first collisions
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kGeoThreads) void geometry_kernel(GeoArgs g) {
+// SP: ring slots (n_phi rounded up to a power of two) for a register sort,
+// -1 for the LDS sort of wider rings.  The first-collider walks are latency
+// bound, so the kernel is held to 64 VGPRs for 8 waves per SIMD (8 workgroups
+// of 20 KB fill the CU's 160 KB of LDS); the compiler parks two loop-invariant
+// addresses in scratch, reloaded once per ring group.
+template <int SP>
+__global__ __launch_bounds__(kGeoThreads) __attribute__((amdgpu_waves_per_eu(8)))
+void geometry_kernel(GeoArgs g) {
   __shared__ RotCand cand_lds[kGeoLdsCand];
   __shared__ double sort_key[kSortCap];
   __shared__ int32_t sort_val[kSortCap];
   const int item_id = g.item0 + static_cast<int>(blockIdx.x);
   const int64_t row = static_cast<int64_t>(blockIdx.x) * g.n_c * g.n_phi;
-  geometry_item(g, item_id, g.tau + row, g.perm + row, cand_lds, sort_key, sort_val);
+  geometry_item<SP>(g, item_id, g.tau + row, g.perm + row, cand_lds, sort_key, sort_val);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,7 +473,17 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
 // ---------------------------------------------------------------------------
 hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s) {
   if (n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(geometry_kernel, dim3(n_items), dim3(kGeoThreads), 0, s, a);
+  const dim3 grid(n_items), block(kGeoThreads);
+  switch (a.n_phi_pow2) {
+    case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(geometry_kernel<4>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(geometry_kernel<8>, grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL(geometry_kernel<16>, grid, block, 0, s, a); break;
+    case 32: hipLaunchKernelGGL(geometry_kernel<32>, grid, block, 0, s, a); break;
+    case 64: hipLaunchKernelGGL(geometry_kernel<64>, grid, block, 0, s, a); break;
+    case 128: hipLaunchKernelGGL(geometry_kernel<128>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(geometry_kernel<-1>, grid, block, 0, s, a); break;
+  }
   return hipGetLastError();
 }
 

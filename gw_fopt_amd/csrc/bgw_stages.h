@@ -110,11 +110,36 @@ __device__ __forceinline__ bool wave_any(bool p) { return __any(p ? 1 : 0) != 0;
 
 namespace {
 
+// Exchange with lane ^ m for m = 1, 2, 4, 8 (partners in the same 16-lane
+// row) by DPP moves, which stay in the VALU instead of going through the LDS
+// crossbar as ds_bpermute does: xor 1 and 2 are quad permutations
+// (quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E), xor 8 rotates the row by 8
+// (row_ror:8 = 0x128), and xor 4 fills banks 0 and 2 from 4 lanes up
+// (row_shl:4 = 0x104, bank mask 0101) and banks 1 and 3 from 4 lanes down
+// (row_shr:4 = 0x114, bank mask 1010).  Every lane of the row must be active.
+__device__ __forceinline__ int xor_lanes(int v, int m) {
+  if (m == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+  if (m == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+  if (m == 4) {
+    const int t = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xF, 0x5, false);
+    return __builtin_amdgcn_update_dpp(t, v, 0x114, 0xF, 0xA, false);
+  }
+  return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ double xor_lanes(double v, int m) {
+  const long long b = __double_as_longlong(v);
+  const int lo = xor_lanes(static_cast<int>(b), m);
+  const int hi = xor_lanes(static_cast<int>(b >> 32), m);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 // Bitonic sort of the P slots of one ring held by one wavefront: lane l
 // holds ring-local elements l + 64 r (r < P / 64) when P >= 64, or element
 // l % P of ring l / P when P < 64.  Partners less than 64 apart are exchanged
-// across lanes by shuffles, farther ones inside a lane; no LDS traffic and no
-// workgroup barrier.  Keys are (delay, phi index) pairs, all distinct.
+// across lanes (up to 8 apart by DPP moves, 16 and 32 apart by shuffles),
+// farther ones inside a lane; no LDS traffic and no workgroup barrier.  Keys
+// are (delay, phi index) pairs, all distinct.
 template <int P>
 __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, int gcount, int i0,
                              int n_phi, double* tau_out, uint16_t* perm_out) {
@@ -162,8 +187,8 @@ __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, in
 #pragma unroll
           for (int r = 0; r < EPL; ++r) {
             const int e = e0 + 64 * r;
-            const double pk = __shfl_xor(key[r], j, 64);
-            const int32_t pv = __shfl_xor(val[r], j, 64);
+            const double pk = j <= 8 ? xor_lanes(key[r], j) : __shfl_xor(key[r], j, 64);
+            const int32_t pv = j <= 8 ? xor_lanes(val[r], j) : __shfl_xor(val[r], j, 64);
             const bool lower = (e & j) == 0;
             const bool up = (e & k) == 0;
             const bool mine_greater = key[r] > pk || (key[r] == pk && val[r] > pv);
@@ -227,8 +252,20 @@ __device__ void sort_rings_lds(int P, double* sort_key, int32_t* sort_val, int g
   }
 }
 
+// SP > 0: rings of exactly SP slots; SP < 0: the LDS network; SP == 0: picked
+// from P at run time.  A kernel that fixes SP carries only one network, which
+// keeps its register budget to that network's (the run-time switch over all of
+// them costs the first-collider scan a third of its occupancy).
+template <int SP>
 __device__ void sort_rings(int P, double* sort_key, int32_t* sort_val, int gcount, int i0,
                            int n_phi, double* tau_out, uint16_t* perm_out) {
+  if constexpr (SP > 0) {
+    sort_rings_p<SP>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out);
+    return;
+  } else if constexpr (SP < 0) {
+    sort_rings_lds(P, sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out);
+    return;
+  }
   switch (P) {
     case 2: sort_rings_p<2>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
     case 4: sort_rings_p<4>(sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out); break;
@@ -244,7 +281,9 @@ __device__ void sort_rings(int P, double* sort_key, int32_t* sort_val, int gcoun
 // One item's first colliders and sorted per-ring delays, written to
 // tau_out / perm_out (n_c * n_phi).  Called by the whole workgroup; the
 // LDS arrays hold kGeoLdsCand candidates and kSortCap sort slots.  Ends
-// with a barrier.
+// with a barrier.  SP: the ring sort, as for sort_rings (SP > 0 requires
+// g.n_phi_pow2 == SP).
+template <int SP = 0>
 __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
                               uint16_t* perm_out, RotCand* cand_lds, double* sort_key,
                               int32_t* sort_val) {
@@ -433,7 +472,7 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
     // wavefront in registers (P <= 128).  Padding slots carry phi index >=
     // n_phi, so they sort after the directions that never collide (delay +inf
     // as well).
-    sort_rings(P, sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out);
+    sort_rings<SP>(P, sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out);
     __syncthreads();
   }
 }
