@@ -52,6 +52,36 @@ def test_reference_c_abi_config():
     assert rel_to_max(C, Co) <= FP64_REL
 
 
+def test_reference_bench_configuration():
+    """The reference's criterion benchmark case (rust/benches/bulk_flow_bench.rs:8-44):
+    five simultaneous bubbles, 50 x 100 grid (rings of 100 azimuths in 128 sort slots),
+    sets [[0], [1]] / [[3], [3]], 120 omegas geomspace(1e-2, 1e2), t in [0, 15], 800 steps.
+    First colliders bit-exact for every bubble; C within 1e-10 of the largest entry of
+    each (polarisation, set) row; S per bin within 1e-10 (blocked form 1e-9)."""
+    inter = np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0], [0.0, 4.0, 0.0, 4.0],
+                      [0.0, 4.0, 2.0, 4.0], [0.0, 0.0, 2.0, 1.0]])
+    coef, pows = [[0.0], [1.0]], [[3.0], [3.0]]
+    w = np.geomspace(1e-2, 1e2, 120)
+    bf = flow(inter, None, 50, 100, coef, pows)
+    for a in range(len(inter)):
+        first = bf.compute_first_colliding_bubble(a)
+        fo, dto = oracle.first_colliding(inter, None, 50, 100, a)
+        np.testing.assert_array_equal(first, fo)
+        assert bf.compute_delta_tab(a, first).tobytes() == dto.tobytes()
+    C = bf.compute_c_integral(w, t_begin=0.0, t_end=15.0, n_t=800)
+    Co = oracle.compute_c_integral(inter, None, 50, 100, coef, pows, w, t_begin=0.0, t_end=15.0,
+                                   n_t=800)
+    assert C.shape == (2, 2, 120)
+    for pm in range(2):
+        for s in range(2):
+            assert rel_to_max(C[pm, s], Co[pm, s]) <= FP64_REL, (pm, s)
+    k = mb.fibonacci_directions(2)
+    So = oracle.spectrum(inter, None, k, 50, 100, coef, pows, w, t_begin=0.0, t_end=15.0, n_t=800)
+    for precision, bar in (("fp64", FP64_REL), ("fp64_blocked", 1e-9)):
+        S, _ = bf.compute_spectrum(w, k, t_begin=0.0, t_end=15.0, n_t=800, precision=precision)
+        assert rel_per_bin(S, So) <= bar, precision
+
+
 def test_masks_bitexact_plain_8():
     wl = configs.plain_8()
     bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
@@ -242,6 +272,24 @@ def test_masks_bitexact_dense_rotated(seed):
             fo, do = oracle.first_colliding(b, ext, 24, 48, a, k_hat=k)
             np.testing.assert_array_equal(f, fo)
             assert d.tobytes() == do.tobytes()
+
+
+@pytest.mark.parametrize("n_phi", [2, 3, 4, 6, 12, 20, 33, 64, 100, 128])
+def test_masks_bitexact_every_sort_width(n_phi):
+    """Every register sort network of the geometry kernel (rings padded to 2, 4, ..., 128
+    slots, one kernel instance each), padded and unpadded widths."""
+    inter, ext = small_async(29, n=6)
+    bf = flow(inter, ext, 5, n_phi, [[1.0]], [[5.0]])
+    k = mb.fibonacci_directions(3)[2]
+    for a in range(len(inter)):
+        f, d = bf.first_colliding_in_frame(a, k)
+        fo, do = oracle.first_colliding(inter, ext, 5, n_phi, a, k_hat=k)
+        np.testing.assert_array_equal(f, fo)
+        assert d.tobytes() == do.tobytes()
+    w = np.geomspace(0.5, 40.0, 6)
+    S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision="fp64")
+    So = oracle.spectrum(inter, ext, [k], 5, n_phi, [[1.0]], [[5.0]], w, t_end=0.9, n_t=40)
+    assert rel_per_bin(S, So) <= FP64_REL
 
 
 @pytest.mark.parametrize("n_phi", [200, 300, 700])
