@@ -25,7 +25,9 @@ struct GeoArgs {
   uint16_t* perm;         // (n_items, n_c * n_phi), phi index of each sorted entry
   int32_t* first_out;     // optional (n_items, n_c * n_phi), unsorted
   double* dtab_out;       // optional (n_items, n_c * n_phi), unsorted
-  int item0;              // workgroup b works on item item0 + b, tau / perm row b
+  int item0;              // first item of the launch; tau / perm row b is item item0 + b
+  int rings_per_block;    // set by launch_geometry: workgroups split an item's rings
+  int n_split;            //   into n_split ranges of rings_per_block rings
 };
 
 struct SpecArgs {

@@ -49,9 +49,14 @@ void geometry_kernel(GeoArgs g) {
   __shared__ RotCand cand_lds[kGeoLdsCand];
   __shared__ double sort_key[kSortCap];
   __shared__ int32_t sort_val[kSortCap];
-  const int item_id = g.item0 + static_cast<int>(blockIdx.x);
-  const int64_t row = static_cast<int64_t>(blockIdx.x) * g.n_c * g.n_phi;
-  geometry_item<SP>(g, item_id, g.tau + row, g.perm + row, cand_lds, sort_key, sort_val);
+  // workgroup (b * n_split + part): rings [part * rings_per_block, ...) of item b
+  const int b = static_cast<int>(blockIdx.x) / g.n_split;
+  const int part = static_cast<int>(blockIdx.x) - b * g.n_split;
+  const int r0 = part * g.rings_per_block;
+  const int r1 = min(g.n_c, r0 + g.rings_per_block);
+  const int item_id = g.item0 + b;
+  const int64_t row = static_cast<int64_t>(b) * g.n_c * g.n_phi;
+  geometry_item<SP>(g, item_id, g.tau + row, g.perm + row, cand_lds, sort_key, sort_val, r0, r1);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,8 +233,9 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const double* __restric
 // same derivatives are summed directly over the collided directions.
 //
 // Lanes own (block of RB consecutive steps, omega class m mod M); the phase of
-// step t0 + r is exp(i w psi(t0)) E^r with E = exp(i w step (1 - cos th)), by
-// an FP64 recurrence.  The M partial sums of a block are combined by
+// step t0 + r is exp(i w psi(t0)) E^r with E = exp(i w step (1 - cos th)), and
+// Re(conj(G) exp(i w psi)) follows it by a three-term (Chebyshev) recurrence in
+// FP64.  The M partial sums of a block are combined by
 // cross-lane shuffles in a fixed order and every lane keeps its own gradient
 // partials, summed in thread order at the end: the result is deterministic.
 namespace {
@@ -286,6 +292,7 @@ size_t adjoint_lds_bytes(const SpecArgs& a, int threads) {
 
 template <int NS, int RB>
 __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
+  static_assert(RB >= 2, "the phase recurrence starts from two steps");
   extern __shared__ double smem[];
   const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
   const Item it = g.items[item_id];
@@ -368,24 +375,32 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
           const double t0 = ri.t_begin + ri.t_step * static_cast<double>(k0);
           const double psi0 = (t0 - it.z_a) - (t0 - it.t_a) * c_i;
           for (int j = m; j < g.n_w; j += M) {
+            // x_r = Re(conj(G) z E^r) with z = exp(i w psi(t0)), E = exp(i theta):
+            // x_0 and x_1 from z and z E, then x_{r+1} = 2 cos(theta) x_r - x_{r-1}
+            // (one FMA per step and component instead of a complex product and
+            // two FMAs).  The recurrence's rounding grows at most like r^2 eps
+            // for theta -> 0, i.e. below 1e-13 over a block.
             double zi, zr;
             sincos_f64(A.om[j] * psi0, &zi, &zr);
             const double er = A.E[2 * j], ei = A.E[2 * j + 1];
-            double gr[2 * NS], gi[2 * NS];
+            const double z1r = __builtin_fma(zr, er, -(zi * ei));
+            const double z1i = __builtin_fma(zr, ei, zi * er);
+            const double two_c = 2.0 * er;
 #pragma unroll
             for (int c = 0; c < 2 * NS; ++c) {
-              gr[c] = A.G[(j * 2 * NS + c) * 2];
-              gi[c] = A.G[(j * 2 * NS + c) * 2 + 1];
-            }
+              const double gr = A.G[(j * 2 * NS + c) * 2];
+              const double gi = A.G[(j * 2 * NS + c) * 2 + 1];
+              double x0 = __builtin_fma(gr, zr, gi * zi);
+              double x1 = __builtin_fma(gr, z1r, gi * z1i);
+              w[0][c] += x0;
+              w[1][c] += x1;
 #pragma unroll
-            for (int r = 0; r < RB; ++r) {
-#pragma unroll
-              for (int c = 0; c < 2 * NS; ++c)
-                w[r][c] = __builtin_fma(gr[c], zr, __builtin_fma(gi[c], zi, w[r][c]));
-              const double nr = __builtin_fma(zr, er, -(zi * ei));
-              const double ni = __builtin_fma(zr, ei, zi * er);
-              zr = nr;
-              zi = ni;
+              for (int r = 2; r < RB; ++r) {
+                const double x2 = __builtin_fma(two_c, x1, -x0);
+                w[r][c] += x2;
+                x0 = x1;
+                x1 = x2;
+              }
             }
           }
         }
@@ -471,9 +486,44 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s) {
+// Workgroups per CU the geometry kernel keeps resident (64 VGPRs, 20 KB of LDS).
+constexpr int kGeoResident = 8;
+
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+hipError_t launch_geometry(const GeoArgs& a_in, int n_items, hipStream_t s) {
   if (n_items == 0) return hipSuccess;
-  const dim3 grid(n_items), block(kGeoThreads);
+  // An item's cost varies by an order of magnitude between bubbles (an early
+  // bubble walks long candidate prefixes), so with few workgroups per resident
+  // slot the slowest items set the kernel time.  Each item's rings are split
+  // over several workgroups (whole patch rows of 64 / min(P, 8) rings) until
+  // there are about 64 workgroups per resident slot or a workgroup holds one
+  // patch row (measured at plain_256: 4.9 ms unsplit, 4.4 / 4.0 / 3.98 ms with
+  // 2 / 4 / 8 workgroups per item; grad_128: 1.35 -> 0.74 ms).
+  GeoArgs a = a_in;
+  const int ph = 64 / (a.n_phi_pow2 < 8 ? a.n_phi_pow2 : 8);
+  const int64_t target = static_cast<int64_t>(device_cus()) * kGeoResident * 64;
+  int n_split = 1;
+  while (static_cast<int64_t>(n_items) * n_split * 2 <= target && a.n_c / (n_split * 2) >= ph)
+    n_split *= 2;
+  a.rings_per_block = (a.n_c + n_split - 1) / n_split;
+  a.rings_per_block = (a.rings_per_block + ph - 1) / ph * ph;
+  a.n_split = (a.n_c + a.rings_per_block - 1) / a.rings_per_block;
+  const int64_t blocks = static_cast<int64_t>(n_items) * a.n_split;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  const dim3 grid(static_cast<unsigned int>(blocks)), block(kGeoThreads);
   switch (a.n_phi_pow2) {
     case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, block, 0, s, a); break;
     case 4: hipLaunchKernelGGL(geometry_kernel<4>, grid, block, 0, s, a); break;
@@ -575,8 +625,9 @@ hipError_t launch_adjoint(const SpecArgs& a, int n_items, hipStream_t s) {
   const size_t lds = adjoint_lds_bytes(a, threads);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   switch (a.n_sets) {
-    case 1: return launch_adj_t<1, 16>(a, n_items, threads, lds, s);
-    case 2: return launch_adj_t<2, 8>(a, n_items, threads, lds, s);
+    // blocks of 8 (4) steps: the kernel stays at 3 waves per SIMD
+    case 1: return launch_adj_t<1, 8>(a, n_items, threads, lds, s);
+    case 2: return launch_adj_t<2, 4>(a, n_items, threads, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

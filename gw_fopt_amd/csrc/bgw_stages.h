@@ -278,15 +278,17 @@ __device__ void sort_rings(int P, double* sort_key, int32_t* sort_val, int gcoun
   }
 }
 
-// One item's first colliders and sorted per-ring delays, written to
-// tau_out / perm_out (n_c * n_phi).  Called by the whole workgroup; the
+// One item's first colliders and sorted per-ring delays for rings
+// [ring_begin, ring_end) (all of them by default), written to tau_out /
+// perm_out (the item's n_c * n_phi row).  Called by the whole workgroup; the
 // LDS arrays hold kGeoLdsCand candidates and kSortCap sort slots.  Ends
 // with a barrier.  SP: the ring sort, as for sort_rings (SP > 0 requires
 // g.n_phi_pow2 == SP).
 template <int SP = 0>
 __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
                               uint16_t* perm_out, RotCand* cand_lds, double* sort_key,
-                              int32_t* sort_val) {
+                              int32_t* sort_val, int ring_begin = 0, int ring_end = -1) {
+  if (ring_end < 0) ring_end = g.n_c;
   const Item it = g.items[item_id];
   const RealInfo ri = g.real[it.r];
   const int n_cand = ri.n_tot - 1;
@@ -314,8 +316,8 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
   const int pw = P < 8 ? P : 8;
   const int ph = 64 / pw;
   const int patches_per_row = P / pw;
-  for (int i0 = 0; i0 < g.n_c; i0 += G) {
-    const int gcount = min(G, g.n_c - i0);
+  for (int i0 = ring_begin; i0 < ring_end; i0 += G) {
+    const int gcount = min(G, ring_end - i0);
     const int nslot = gcount * P;
     const int n_patch = ((gcount + ph - 1) / ph) * patches_per_row;
     const int n_multi = (n_patch + D - 1) / D;
