@@ -55,6 +55,8 @@ struct SpecArgs {
   int mixed_block;        // time steps per factorised phase block (8 or 16; 0: default)
   int threads;            // workgroup size (0: default); lanes own omegas
   int time_tables;        // per-item time tables in LDS (spectrum_time_tables)
+  int time_chunk;         // time steps per chunk of weights in LDS: 512, else 256
+                          //   (launch_spectrum picks it)
   double* partial;        // (n_items, 2, n_sets_out, n_w) complex
   int item0;              // tau / perm / partial rows are item - item0
   // adjoint only

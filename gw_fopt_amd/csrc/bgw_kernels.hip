@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void integrand_kernel(SpecArgs g) {
   const int item_id = g.order ? g.order[blockIdx.x] : static_cast<int>(blockIdx.x);
   const Item it = g.items[item_id];
   const RealInfo ri = g.real[it.r];
-  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, false);
+  const int chunk = time_chunk_of(g);
+  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, false, chunk);
   const int64_t slab = static_cast<int64_t>(g.n_t) * g.n_w;  // one (pm, s)
   double* out = g.partial + static_cast<int64_t>(blockIdx.x) * 2 * g.n_sets_out * slab * 2;
   double om[OPL];
@@ -171,8 +172,8 @@ __global__ __launch_bounds__(256) void integrand_kernel(SpecArgs g) {
       __syncthreads();
       continue;
     }
-    for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
-      const int len = min(kTimeChunk, g.n_t - t0);
+    for (int t0 = t_lo; t0 < g.n_t; t0 += chunk) {
+      const int len = min(chunk, g.n_t - t0);
       prepare_times<false, NS>(g, L, i, t0, len, it, ri);
       __syncthreads();
       for (int l = 0; l < len; ++l) {
@@ -543,18 +544,47 @@ bool spectrum_time_tables(const SpecArgs& a) {
 
 size_t spectrum_lds_bytes(const SpecArgs& a, int /*mode*/) {
   const int nterms = a.n_sets * a.n_terms;
-  size_t doubles = ring_lds_doubles(a.n_phi, nterms) + kTimeChunk +
-                   static_cast<size_t>(kTimeChunk) * 2 * a.n_sets;
+  const int chunk = time_chunk_of(a);
+  size_t doubles = ring_lds_doubles(a.n_phi, nterms) + chunk +
+                   static_cast<size_t>(chunk) * 2 * a.n_sets;
   if (a.time_tables) doubles += time_table_doubles(a.n_t, nterms);
   return doubles * sizeof(double);
 }
 
 namespace {
+int resident_blocks(const void* k, int threads, size_t lds) {
+  int nb = 0;
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, threads, lds) != hipSuccess)
+    return 0;
+  return nb;
+}
+
+// The chunk length: 512 steps (half the anchors and barriers of 256) when the
+// run is longer than 256 steps and the larger chunk costs no resident
+// workgroups; 256 otherwise.  Measured: plain_256 blocked 27.5 -> 26.8 ms with
+// 512; ensemble_512 (64-lane workgroups, LDS-bound occupancy) 6-17 % slower
+// with 512, so it keeps 256.
 template <int NS, int OPL, int MODE, int RB>
-hipError_t launch_spec_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
+hipError_t launch_spec_t(const SpecArgs& a_in, int n_items, int threads, size_t /*lds*/,
+                         hipStream_t s) {
   auto k = spectrum_kernel<NS, OPL, MODE, RB>;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+  const void* kp = reinterpret_cast<const void*>(k);
+  SpecArgs a = a_in;
+  a.time_chunk = kTimeChunkMin;
+  size_t lds = spectrum_lds_bytes(a, MODE);
+  if (a.n_t > kTimeChunkMin) {
+    SpecArgs b = a;
+    b.time_chunk = kTimeChunkMax;
+    const size_t lds_b = spectrum_lds_bytes(b, MODE);
+    if (lds_b <= 160 * 1024 && resident_blocks(kp, threads, lds_b) >= resident_blocks(kp, threads, lds) &&
+        resident_blocks(kp, threads, lds_b) > 0) {
+      a = b;
+      lds = lds_b;
+    }
+  }
+  hipError_t e = hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(n_items), dim3(threads), lds, s, a);

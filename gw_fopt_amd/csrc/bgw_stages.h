@@ -521,7 +521,14 @@ __device__ __forceinline__ double cutoff_weight(const SpecArgs& g, double t) {
   return exp(-(d * d) * g.inv_t0_sq);
 }
 
-constexpr int kTimeChunk = 256;
+// Time steps per chunk of per-step weights in LDS (SpecArgs::time_chunk): one
+// exact anchor per chunk and omega in the blocked form.  Both lengths are
+// multiples of the mixed form's anchor groups (16 blocks of 16 steps).
+constexpr int kTimeChunkMin = 256;
+constexpr int kTimeChunkMax = 512;
+__host__ __device__ inline int time_chunk_of(const SpecArgs& a) {
+  return a.time_chunk == kTimeChunkMax ? kTimeChunkMax : kTimeChunkMin;
+}
 // Blocked forms: an exact anchor sincos every kReanchor blocks of a chunk.
 constexpr int kReanchor = 16;
 
@@ -539,8 +546,8 @@ struct RingLds {
   double* cls;  // [NT][n_phi + 1]
   double* tb;   // [n_t] K dt w_t cutoff(t) (t - t_a)^3, or null
   double* tf;   // [NT][n_t] coef_q (t - t_a)^-p_q, or null
-  double* psi;  // [kTimeChunk] phase argument per time step
-  double* v;    // [kTimeChunk][2 * n_sets] weights V+ / Vx per time step
+  double* psi;  // [time chunk] phase argument per time step
+  double* v;    // [time chunk][2 * n_sets] weights V+ / Vx per time step
   float* vf;    // the same weights in float (mixed form), aliases v
   int stride;   // n_phi + 1
 };
@@ -554,7 +561,8 @@ __host__ __device__ inline size_t time_table_doubles(int n_t, int nt_terms) {
   return static_cast<size_t>(n_t) * (1 + nt_terms);
 }
 
-__device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool time_tables) {
+__device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool time_tables,
+                        int chunk) {
   RingLds L;
   L.stride = n_phi + 1;
   double* p = smem;
@@ -573,7 +581,7 @@ __device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool ti
     L.tb = p; p += n_t;
     L.tf = p; p += static_cast<size_t>(nt_terms) * n_t;
   }
-  L.psi = p; p += kTimeChunk;
+  L.psi = p; p += chunk;
   L.v = p;
   L.vf = reinterpret_cast<float*>(p);
   return L;
@@ -686,7 +694,8 @@ __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0
   constexpr int ns = NS;
   const double c_i = g.ring_c[i];
   const double rf = g.ring_fac[i];
-  for (int l = threadIdx.x; l < kTimeChunk; l += blockDim.x) {
+  const int chunk = time_chunk_of(g);
+  for (int l = threadIdx.x; l < chunk; l += blockDim.x) {
     double vv[2 * NS];
 #pragma unroll
     for (int s = 0; s < 2 * ns; ++s) vv[s] = 0.0;
@@ -783,7 +792,8 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
   using Acc = typename std::conditional<MODE == 2, float, double>::type;
   const Item it = g.items[item_id];
   const RealInfo ri = g.real[it.r];
-  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, g.time_tables != 0);
+  const int chunk = time_chunk_of(g);
+  const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, g.time_tables != 0, chunk);
   if (L.tb) prepare_item_tables(g, L, it, ri);
   __syncthreads();
 
@@ -842,8 +852,8 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
         ebi[q] = ci;
       }
     }
-    for (int t0 = t_lo; t0 < g.n_t; t0 += kTimeChunk) {
-      const int len = min(kTimeChunk, g.n_t - t0);
+    for (int t0 = t_lo; t0 < g.n_t; t0 += chunk) {
+      const int len = min(chunk, g.n_t - t0);
       prepare_times<MODE == 2, NS>(g, L, i, t0, len, it, ri);
       __syncthreads();
       if constexpr (MODE == 0) {

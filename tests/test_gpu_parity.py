@@ -208,6 +208,27 @@ def test_spectrum_plain_8_mixed():
     assert np.median(rel) <= MIXED_REL
 
 
+def test_spectrum_multi_chunk_blocked_and_mixed():
+    """1100 time steps: two full chunks of 512 steps and a partial one, for the blocked
+    form against the oracle and the mixed form against FP64."""
+    wl = configs.plain_8()
+    k = wl.k_hat[:4]
+    bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
+              wl.powers_sets)
+    So = oracle.spectrum(wl.interior, wl.exterior, k, wl.n_cos_thetax, wl.n_phix,
+                         wl.coefficients_sets, wl.powers_sets, wl.w_arr, t_end=wl.t_end, n_t=1100)
+    S64, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100)
+    assert rel_per_bin(S64, So) <= FP64_REL
+    Sb, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="fp64_blocked")
+    assert rel_per_bin(Sb, So) <= 1e-9
+    S32, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="mixed")
+    rel = np.abs(S32 - S64) / np.abs(S64)
+    # 5.5x more time steps than plain_8: the cancellation floor of the float
+    # weights rises with the number of terms (3.2e-5 measured)
+    assert rel.max() <= 1e-4
+    assert np.median(rel) <= MIXED_REL
+
+
 def test_identity_frame_equals_reference_frame():
     """k-hat = z-hat reproduces the reference-frame call bit for bit."""
     wl = configs.plain_8()
