@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
             const double wp = Wt[s], wx = Wt[NS + s];
             for (int kk = 0; kk < g.n_terms; ++kk) {
               const int q = s * g.n_terms + kk;
-              const double dmp = pow(dta, -g.pows[q]);
+              const double dmp = pow_param(dta, -g.pows[q]);
               const double pp = L.cc[q * L.stride + cnt], px = L.cs[q * L.stride + cnt];
               const double lp = L.cl[q * L.stride + cnt], lx = L.cls[q * L.stride + cnt];
               A.acc[(2 * q) * bd + tid] += base * dmp * (wp * pp + wx * px);
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
               for (int r = 0; r < cnt; ++r) {
                 const double ratio = L.tau[r] / dta;
                 const double d = exp((-dta) * (1.0 - ratio) / g.damping);
-                const double rp = pow(ratio, g.pows[q]) * d;
+                const double rp = pow_param(ratio, g.pows[q]) * d;
                 const double rl = rp * log(ratio);
                 dcp = __builtin_fma(L.gc[r], rp, dcp);
                 dcx = __builtin_fma(L.gs[r], rp, dcx);

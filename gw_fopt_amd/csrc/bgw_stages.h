@@ -505,6 +505,25 @@ __device__ void wave_scan(double* arr, int n, bool reverse) {
   }
 }
 
+// x^p for a scaling power p: exact-integer exponents |p| <= 32 (the usual
+// bulk-flow powers 3, 4, 5, ...) by repeated squaring -- at most 2 log2|p|
+// roundings, a few ulp, against ~100 instructions of the general pow -- and
+// pow otherwise.  The branch is uniform: p is a parameter of the call.
+__device__ __forceinline__ double pow_param(double x, double p) {
+  const double ap = fabs(p);
+  if (ap <= 32.0 && ap == __builtin_rint(ap)) {
+    int n = static_cast<int>(ap);
+    double r = 1.0, b = x;
+    while (n) {
+      if (n & 1) r *= b;
+      n >>= 1;
+      if (n) b *= b;
+    }
+    return p < 0.0 ? 1.0 / r : r;
+  }
+  return pow(x, p);
+}
+
 __device__ __forceinline__ int count_le(const double* sorted, int n, double x) {
   int lo = 0, hi = n;
   while (lo < hi) {
@@ -606,7 +625,7 @@ __device__ int prepare_ring(const SpecArgs& g, const RingLds& L, int64_t tau_ite
     L.us[r] = gsv;
     const bool fin = tau < kInf;
     for (int q = 0; q < nterms; ++q) {
-      const double pw = fin ? pow(tau, g.pows[q]) : 0.0;
+      const double pw = fin ? pow_param(tau, g.pows[q]) : 0.0;
       L.cc[q * L.stride + r + 1] = gcv * pw;
       L.cs[q * L.stride + r + 1] = gsv * pw;
       if (L.cl) {
@@ -681,7 +700,7 @@ __device__ void prepare_item_tables(const SpecArgs& g, const RingLds& L, const I
     }
     L.tb[k] = base;
     for (int q = 0; q < nterms; ++q)
-      L.tf[static_cast<size_t>(q) * g.n_t + k] = dta > 0.0 ? g.coef[q] * pow(dta, -g.pows[q]) : 0.0;
+      L.tf[static_cast<size_t>(q) * g.n_t + k] = dta > 0.0 ? g.coef[q] * pow_param(dta, -g.pows[q]) : 0.0;
   }
 }
 
@@ -721,7 +740,7 @@ __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0
             for (int kk = 0; kk < g.n_terms; ++kk) {
               const int q = s * g.n_terms + kk;
               const double f = L.tf ? L.tf[static_cast<size_t>(q) * g.n_t + k]
-                                    : g.coef[q] * pow(dta, -g.pows[q]);
+                                    : g.coef[q] * pow_param(dta, -g.pows[q]);
               bp = __builtin_fma(f, L.cc[q * L.stride + cnt], bp);
               bx = __builtin_fma(f, L.cs[q * L.stride + cnt], bx);
             }
@@ -743,7 +762,7 @@ __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0
                 f = 0.0;
                 for (int kk = 0; kk < g.n_terms; ++kk) {
                   const int q = s * g.n_terms + kk;
-                  f += g.coef[q] * pow(ratio, g.pows[q]);
+                  f += g.coef[q] * pow_param(ratio, g.pows[q]);
                 }
                 f *= exp((-dta) * (1.0 - ratio) / g.damping);
               }
