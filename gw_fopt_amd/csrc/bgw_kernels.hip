@@ -12,7 +12,10 @@
 //   (a ballot), first to find the minimum-delay candidate, then to verify it.
 //   A lane whose candidate fails verification, or that meets non-finite data,
 //   re-runs the reference's own full scan, so decisions are bit-exact.  Each
-//   ring's delays are then sorted in LDS (bitonic) for stage 2.
+//   ring's delays are then sorted for stage 2 (a bitonic network in registers,
+//   one ring per wavefront; in LDS for rings of more than 128 slots).  A
+//   workgroup takes a range of an item's rings (launch_geometry splits items
+//   so that uneven items balance).
 //
 // Stage 2 (spectrum_kernel): per item, ring by ring, the phi integral
 //   B_pm(t) of compute_b_integral (:481-586) is read off prefix sums over the
@@ -22,6 +25,9 @@
 //   accumulate C_pm(omega) = sum_t V_pm(t) exp(i omega psi(t)), with
 //   psi = (t - z_a) - (t - t_a) cos th, in registers:
 //     FP64 form:  one double sincos per (t, omega) term;
+//     blocked FP64 form: the phase of a block of R steps is anchor * E_r in
+//                 double, blocks combined by Horner's rule, one exact anchor
+//                 per chunk of steps;
 //     mixed form: psi arguments reduced in double; the phase of a block of
 //                 R consecutive steps factorised as anchor * E_r with
 //                 E_r = exp(i omega r dt (1 - cos th)); float sincos and
