@@ -318,7 +318,6 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
   const int patches_per_row = P / pw;
   for (int i0 = ring_begin; i0 < ring_end; i0 += G) {
     const int gcount = min(G, ring_end - i0);
-    const int nslot = gcount * P;
     const int n_patch = ((gcount + ph - 1) / ph) * patches_per_row;
     const int n_multi = (n_patch + D - 1) / D;
     const int nwork_round = ((n_multi * 64 + blockDim.x - 1) / blockDim.x) * blockDim.x;
