@@ -313,6 +313,24 @@ def test_masks_bitexact_every_sort_width(n_phi):
     assert rel_per_bin(S, So) <= FP64_REL
 
 
+@pytest.mark.parametrize("n_c", [9, 17, 33, 64])
+def test_masks_bitexact_split_rings(n_c):
+    """Few items, so each item's rings are split over several workgroups (ragged last
+    range); first colliders, delays and the spectrum against the oracle."""
+    inter, ext = small_async(31, n=4)
+    bf = flow(inter, ext, n_c, 20, [[1.0]], [[5.0]])
+    k = mb.fibonacci_directions(3)[1]
+    for a in range(len(inter)):
+        f, d = bf.first_colliding_in_frame(a, k)
+        fo, do = oracle.first_colliding(inter, ext, n_c, 20, a, k_hat=k)
+        np.testing.assert_array_equal(f, fo)
+        assert d.tobytes() == do.tobytes()
+    w = np.geomspace(0.5, 40.0, 5)
+    S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision="fp64")
+    So = oracle.spectrum(inter, ext, [k], n_c, 20, [[1.0]], [[5.0]], w, t_end=0.9, n_t=40)
+    assert rel_per_bin(S, So) <= FP64_REL
+
+
 @pytest.mark.parametrize("n_phi", [200, 300, 700])
 def test_masks_bitexact_wide_rings(n_phi):
     """Rings of more than 128 azimuths (sorted in LDS: 256, 512 and 1024 slots)."""
