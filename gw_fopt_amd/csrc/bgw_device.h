@@ -34,10 +34,23 @@ __device__ __forceinline__ double lscalar(double u0, double u1, double u2, doubl
 // step rounds once; the split constants are pi/2 rounded to double and the
 // two successive remainders (computed from a 100-digit value of pi).  The
 // reduced argument |r| <= pi/4 (plus rounding) goes into truncated Taylor
-// series: through r^17 for sin and r^16 for cos in double (truncation below
-// 0.1 ulp at pi/4), through r^9 / r^10 in float.  The quadrant q mod 4 picks
-// and negates the two results.
+// series through r^17 for sin and r^16 for cos (truncation below 0.1 ulp at
+// pi/4).  The quadrant q mod 4 picks and negates the two results.
 // ---------------------------------------------------------------------------
+
+// a * b + c as one v_fma_f64 with the addend c in an SGPR pair; c must be
+// uniform (here: a polynomial coefficient).  In Horner steps
+// p = fma(p, r2, coeff) hipcc otherwise selects the two-address v_fmac_f64,
+// which overwrites its addend, and copies the coefficient into a VGPR first, a
+// v_mov_b64 per step.  The three-address form needs no copy and no VGPR for the
+// coefficient: FP64 form's phase sums 177 -> 142 ms at plain_256, adjoint
+// 3.26 -> 3.06 ms at grad_128, blocked and mixed forms unchanged.  Same rounding
+// as __builtin_fma (one fused operation).
+__device__ __forceinline__ double fma_coeff(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+}
 
 __device__ __forceinline__ double reduce_half_pi(double x, int* quadrant) {
   const double two_over_pi = 0x1.45f306dc9c883p-1;
@@ -65,42 +78,23 @@ __device__ __forceinline__ void sincos_f64(double x, double* sin_out, double* co
   double r = reduce_half_pi(x, &n);
   double r2 = r * r;
   double ps = 0x1.952c77030ad4ap-49;                    //  1/17!
-  ps = __builtin_fma(ps, r2, -0x1.ae7f3e733b81fp-41);  // -1/15!
-  ps = __builtin_fma(ps, r2, 0x1.6124613a86d09p-33);   //  1/13!
-  ps = __builtin_fma(ps, r2, -0x1.ae64567f544e4p-26);  // -1/11!
-  ps = __builtin_fma(ps, r2, 0x1.71de3a556c734p-19);   //  1/9!
-  ps = __builtin_fma(ps, r2, -0x1.a01a01a01a01ap-13);  // -1/7!
-  ps = __builtin_fma(ps, r2, 0x1.1111111111111p-7);    //  1/5!
-  ps = __builtin_fma(ps, r2, -0x1.5555555555555p-3);   // -1/3!
+  ps = fma_coeff(ps, r2, -0x1.ae7f3e733b81fp-41);  // -1/15!
+  ps = fma_coeff(ps, r2, 0x1.6124613a86d09p-33);   //  1/13!
+  ps = fma_coeff(ps, r2, -0x1.ae64567f544e4p-26);  // -1/11!
+  ps = fma_coeff(ps, r2, 0x1.71de3a556c734p-19);   //  1/9!
+  ps = fma_coeff(ps, r2, -0x1.a01a01a01a01ap-13);  // -1/7!
+  ps = fma_coeff(ps, r2, 0x1.1111111111111p-7);    //  1/5!
+  ps = fma_coeff(ps, r2, -0x1.5555555555555p-3);   // -1/3!
   double s = __builtin_fma(r * r2, ps, r);
   double pc = 0x1.ae7f3e733b81fp-45;                    //  1/16!
-  pc = __builtin_fma(pc, r2, -0x1.93974a8c07c9dp-37);  // -1/14!
-  pc = __builtin_fma(pc, r2, 0x1.1eed8eff8d898p-29);   //  1/12!
-  pc = __builtin_fma(pc, r2, -0x1.27e4fb7789f5cp-22);  // -1/10!
-  pc = __builtin_fma(pc, r2, 0x1.a01a01a01a01ap-16);   //  1/8!
-  pc = __builtin_fma(pc, r2, -0x1.6c16c16c16c17p-10);  // -1/6!
-  pc = __builtin_fma(pc, r2, 0x1.5555555555555p-5);    //  1/4!
-  pc = __builtin_fma(pc, r2, -0.5);                    // -1/2!
+  pc = fma_coeff(pc, r2, -0x1.93974a8c07c9dp-37);  // -1/14!
+  pc = fma_coeff(pc, r2, 0x1.1eed8eff8d898p-29);   //  1/12!
+  pc = fma_coeff(pc, r2, -0x1.27e4fb7789f5cp-22);  // -1/10!
+  pc = fma_coeff(pc, r2, 0x1.a01a01a01a01ap-16);   //  1/8!
+  pc = fma_coeff(pc, r2, -0x1.6c16c16c16c17p-10);  // -1/6!
+  pc = fma_coeff(pc, r2, 0x1.5555555555555p-5);    //  1/4!
+  pc = fma_coeff(pc, r2, -0.5);                    // -1/2!
   double c = __builtin_fma(r2, pc, 1.0);
-  apply_quadrant(n, s, c, sin_out, cos_out);
-}
-
-// Reduced argument in double, polynomial in float.
-__device__ __forceinline__ void sincos_f64_to_f32(double x, float* sin_out, float* cos_out) {
-  int n;
-  float r = static_cast<float>(reduce_half_pi(x, &n));
-  float r2 = r * r;
-  float ps = 0x1.71de3ap-19f;                      //  1/9!
-  ps = __builtin_fmaf(ps, r2, -0x1.a01a02p-13f);  // -1/7!
-  ps = __builtin_fmaf(ps, r2, 0x1.111112p-7f);    //  1/5!
-  ps = __builtin_fmaf(ps, r2, -0x1.555556p-3f);   // -1/3!
-  float s = __builtin_fmaf(r * r2, ps, r);
-  float pc = -0x1.27e4fcp-22f;                     // -1/10!
-  pc = __builtin_fmaf(pc, r2, 0x1.a01a02p-16f);   //  1/8!
-  pc = __builtin_fmaf(pc, r2, -0x1.6c16c2p-10f);  // -1/6!
-  pc = __builtin_fmaf(pc, r2, 0x1.555556p-5f);    //  1/4!
-  pc = __builtin_fmaf(pc, r2, -0.5f);             // -1/2!
-  float c = __builtin_fmaf(r2, pc, 1.0f);
   apply_quadrant(n, s, c, sin_out, cos_out);
 }
 

@@ -30,8 +30,9 @@
 //                 per chunk of steps;
 //     mixed form: psi arguments reduced in double; the phase of a block of
 //                 R consecutive steps factorised as anchor * E_r with
-//                 E_r = exp(i omega r dt (1 - cos th)); float sincos and
-//                 float block sums, Kahan-compensated accumulation.
+//                 E_r = exp(i omega r dt (1 - cos th)); exact double anchors
+//                 every 16 blocks, float phase factors E_r, weights and block
+//                 sums, Kahan-compensated float accumulation.
 //
 // Stage 3 (reduce_kernel): sum the per-bubble partial C in bubble order
 //   (deterministic) and form S = 8 pi w^3 (|C+|^2 + |Cx|^2).
