@@ -59,6 +59,11 @@ def parse():
     p.add_argument("--no-nucleation", action="store_true")
     p.add_argument("--profile-nucleation", action="store_true",
                    help="only run the 512-realisation nucleation batch --steps times (for rocprofv3)")
+    p.add_argument("--profile-ensemble", default=None, metavar="FORM",
+                   help="only run ensemble_512 in this form --steps times with the tiles the "
+                        "autotuner picks for it (for rocprofv3)")
+    p.add_argument("--profile-grad", action="store_true",
+                   help="only run --steps grad_128 gradients (forward + adjoint, for rocprofv3)")
     p.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic counters")
     p.add_argument("--modes", default="mixed,fp64_blocked,fp64")
     p.add_argument("--profile-mode", default=None,
@@ -361,6 +366,30 @@ def main():
             _, _, _, _, counts = mb.nucleate_fixed_rate_batch(specs, **NUC_KW)
         print(json.dumps({"profile": "nucleation_512", "kernel_ms": mb.last_nucleation_kernel_ms(),
                           "bubbles": int(counts.sum())}), flush=True)
+        return
+    if args.profile_ensemble:
+        from gw_fopt_amd.ensemble import BubbleEnsemble
+        ens = configs.ensemble_512()
+        E = BubbleEnsemble(ens.interiors, ens.exteriors)
+        E.set_resolution(ens.n_cos_thetax, ens.n_phix)
+        E.set_gradient_scaling_params(ens.coefficients_sets, ens.powers_sets)
+        w = ens.w_arr()
+        tiles = E.autotune(w, ens.k_hat, t_end=ens.t_end, n_t=ens.n_t,
+                           precision=args.profile_ensemble)
+        ms = []
+        for _ in range(args.steps):
+            E.compute_spectrum(w, ens.k_hat, t_end=ens.t_end, n_t=ens.n_t,
+                               precision=args.profile_ensemble, tiles=tiles)
+            ms.append(E.last_timings()["total_ms"])
+        print(json.dumps({"profile": "ensemble_512", "form": args.profile_ensemble, "tiles": tiles,
+                          "device_ms": ms}), flush=True)
+        return
+    if args.profile_grad:
+        from gw_fopt_amd.gradbench import GradProblem
+        P = GradProblem(precision="fp64_blocked")
+        for _ in range(args.steps):
+            P.loss_and_grad((1.8, 1.1))
+        print(json.dumps({"profile": "grad_128", "gradients": args.steps}), flush=True)
         return
     wl = configs.plain_256(n_k=args.n_k)
     if args.profile_mode:
