@@ -415,6 +415,9 @@ void build_static(Engine& E, const std::vector<Bubbles>& reals, size_t n_threads
 struct Request {
   // shared
   const Resolution* res = nullptr;
+  // several surface-weight sets of one grid (the tensor components): k-hat slot
+  // k of the request (identity frame) uses surfaces[k]; empty: res's weights
+  std::vector<const Resolution*> surfaces;
   const Scaling* sc = nullptr;
   Cutoff cutoff;
   size_t n_w = 0, n_t = 0;
@@ -539,9 +542,24 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   upload(E.d_cos_t, res.cos_t, st);
   upload(E.d_cos_p, res.cos_p, st);
   upload(E.d_sin_p, res.sin_p, st);
-  upload(E.d_gcos, res.gcos, st);
-  upload(E.d_gsin, res.gsin, st);
-  upload(E.d_ring_fac, res.ring_fac, st);
+  if (q.surfaces.empty()) {
+    upload(E.d_gcos, res.gcos, st);
+    upload(E.d_gsin, res.gsin, st);
+    upload(E.d_ring_fac, res.ring_fac, st);
+  } else {
+    if (q.surfaces.size() != n_k) throw CalcError("one surface-weight set per k-hat slot");
+    std::vector<double> gc, gs, rf;
+    for (const Resolution* S : q.surfaces) {
+      if (S->n_c != res.n_c || S->n_phi != res.n_phi)
+        throw CalcError("surface-weight sets of different grids");
+      gc.insert(gc.end(), S->gcos.begin(), S->gcos.end());
+      gs.insert(gs.end(), S->gsin.begin(), S->gsin.end());
+      rf.insert(rf.end(), S->ring_fac.begin(), S->ring_fac.end());
+    }
+    upload(E.d_gcos, gc, st);
+    upload(E.d_gsin, gs, st);
+    upload(E.d_ring_fac, rf, st);
+  }
   upload(E.d_omega, q.omega, st);
   upload(E.d_coef, sc.coef, st);
   upload(E.d_pows, sc.pows, st);
@@ -596,6 +614,7 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   sa.damping = sc.damping;
   sa.kfac = 1.0 / (6.0 * kPi);
   sa.skip_uncollided = q.skip_uncollided ? 1 : 0;
+  sa.surfaces = q.surfaces.empty() ? 1 : static_cast<int>(q.surfaces.size());
   sa.mixed_block = q.tile_rb;
   sa.threads = q.tile_threads;
   sa.partial = E.d_partial.as<double>();
@@ -1273,13 +1292,48 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
   return r.c;
 }
 
+// compute_c on several surface-weight sets of the handle's grid in one request:
+// the sets take the k-hat slots of an identity-frame request, so their items run
+// in the same launches.  Returns each set's C, laid out as compute_c's.
+std::vector<std::vector<double>> compute_c_surfaces(bgw_generalized_bulk_flow* h,
+                                                    const double* w_arr, size_t n_w,
+                                                    bool has_t_begin, double t_begin,
+                                                    double t_end, size_t n_t,
+                                                    const std::vector<int32_t>& ids, int mode,
+                                                    const std::vector<Resolution>& surfaces) {
+  const size_t n_sets = h->sc.n_sets;
+  Request q;
+  q.res = &surfaces.front();
+  for (const Resolution& S : surfaces) q.surfaces.push_back(&S);
+  q.sc = &h->sc;
+  q.cutoff = make_cutoff(BgwTimeCutoffConfig{BgwTimeCutoffKind_Unit, 0.0, NAN, NAN});
+  q.n_w = n_w;
+  q.n_t = n_t;
+  q.identity = true;
+  q.k.assign(surfaces.size(), {0.0, 0.0, 1.0});
+  q.omega.assign(w_arr, w_arr + n_w);
+  q.t_begin = {has_t_begin ? t_begin : 0.0};
+  q.t_end = {t_end};
+  q.bubbles_of_real = {ids};
+  q.mode = mode;
+  q.want_s = false;
+  q.skip_uncollided = false;  // components such as xz may be small
+  apply_tiles(h->tiles, q);
+  Result r = run(h->engine, q);
+  const size_t blk = 2 * n_sets * n_w * 2;  // one group's C
+  std::vector<std::vector<double>> out;
+  for (size_t k = 0; k < surfaces.size(); ++k)
+    out.emplace_back(r.c.begin() + k * blk, r.c.begin() + (k + 1) * blk);
+  return out;
+}
+
 // Surface weights of the tensor components (new_generalized_bulk_flow.rs:592-727):
 // the phi-integral of component c on ring i is
 //   sum_j w_j f_j phi_c(phi_j) * theta_c(theta_i) * dphi * dta^3,
 // phi_c in {cos^2, cos sin, cos, sin^2, sin, 1}, theta_c in {sin^2, sin^2,
 // sin cos, sin^2, sin cos, cos^2}, with sin = sqrt(max(1 - cos^2, 0)).  The
-// phase sums take two phi-functions with one theta-factor per launch, so the
-// six components are four launches: (xx, xy), (xz, yz), (yy, -), (zz, -).
+// phase sums take two phi-functions with one theta-factor per weight set, so the
+// six components are four weight sets: (xx, xy), (xz, yz), (yy, -), (zz, -).
 // The integrand path (compute_c_integrand, generalized_bulk_flow.rs:875-920) on the
 // surface weights of `surface` (the handle's own when null).  The two components
 // of the run are written to components dst[0], dst[1] (skipped when < 0) of out,
@@ -1514,19 +1568,19 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_tensor(
     if (!ids.empty()) {
       check_time_range(has_t_begin ? t_begin : 0.0, t_end);
       const Resolution& R = need_resolution(h->res);
-      const BgwTimeCutoffConfig unit = {BgwTimeCutoffKind_Unit, 0.0, NAN, NAN};
-      // launch -> (component of C+, component of Cx), -1: unused
+      // weight set -> (component of C+, component of Cx), -1: unused
       const int comp[4][2] = {{0, 1}, {2, 4}, {3, -1}, {5, -1}};
       const size_t blk = 2 * n_sets * n_w;  // doubles of one component
-      for (int pair = 0; pair < 4; ++pair) {
-        const Resolution S = tensor_surface(R, pair);
-        const std::vector<double> c = compute_c(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
-                                                selected_bubbles, n_selected_bubbles, unit,
-                                                kernel_mode(precision), &S);
+      std::vector<Resolution> surfaces;
+      for (int pair = 0; pair < 4; ++pair) surfaces.push_back(tensor_surface(R, pair));
+      // the four weight sets in one request: their items share the launches
+      const std::vector<std::vector<double>> cs =
+          compute_c_surfaces(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t, ids,
+                             kernel_mode(precision), surfaces);
+      for (int pair = 0; pair < 4; ++pair)
         for (int pm = 0; pm < 2; ++pm)
           if (comp[pair][pm] >= 0)
-            std::memcpy(&res[comp[pair][pm] * blk], &c[pm * blk], blk * sizeof(double));
-      }
+            std::memcpy(&res[comp[pair][pm] * blk], &cs[pair][pm * blk], blk * sizeof(double));
     }
     std::memcpy(out, res.data(), res.size() * sizeof(double));
   });

@@ -57,6 +57,8 @@ struct SpecArgs {
   int time_tables;        // per-item time tables in LDS (spectrum_time_tables)
   int time_chunk;         // time steps per chunk of weights in LDS: 512, else 256
                           //   (launch_spectrum picks it)
+  int surfaces;           // > 1: that many surface-weight sets stacked in gcos / gsin
+                          //   (n_phi each) and ring_fac (n_c each), item k uses set k
   double* partial;        // (n_items, 2, n_sets_out, n_w) complex
   int item0;              // tau / perm / partial rows are item - item0
   // adjoint only

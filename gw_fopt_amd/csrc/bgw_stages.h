@@ -608,15 +608,22 @@ __device__ RingLds carve(double* smem, int n_phi, int nt_terms, int n_t, bool ti
 // Loads ring i of an item (row tau_item of g.tau / g.perm) and builds its
 // prefix/suffix tables.  Returns the
 // first time index with a contribution and whether the ring is skipped.
+// Surface-weight set of an item: the k-hat slot when several sets are stacked.
+__device__ __forceinline__ int surface_of(const SpecArgs& g, const Item& it) {
+  return g.surfaces > 1 ? it.k : 0;
+}
+
 __device__ int prepare_ring(const SpecArgs& g, const RingLds& L, int64_t tau_item, int i,
-                            double t_a, const RealInfo& ri, bool* skip) {
+                            double t_a, const RealInfo& ri, bool* skip, int surf = 0) {
   const int n_phi = g.n_phi;
   const int nterms = g.n_sets * g.n_terms;
   const int64_t base = (tau_item * g.n_c + i) * static_cast<int64_t>(n_phi);
+  const double* gcos = g.gcos + static_cast<int64_t>(surf) * n_phi;
+  const double* gsin = g.gsin + static_cast<int64_t>(surf) * n_phi;
   for (int r = threadIdx.x; r < n_phi; r += blockDim.x) {
     const double tau = g.tau[base + r];
     const int j = g.perm[base + r];
-    const double gcv = g.gcos[j], gsv = g.gsin[j];
+    const double gcv = gcos[j], gsv = gsin[j];
     L.tau[r] = tau;
     L.gc[r] = gcv;
     L.gs[r] = gsv;
@@ -711,7 +718,7 @@ __device__ void prepare_times(const SpecArgs& g, const RingLds& L, int i, int t0
   const int n_phi = g.n_phi;
   constexpr int ns = NS;
   const double c_i = g.ring_c[i];
-  const double rf = g.ring_fac[i];
+  const double rf = g.ring_fac[static_cast<int64_t>(surface_of(g, it)) * g.n_c + i];
   const int chunk = time_chunk_of(g);
   for (int l = threadIdx.x; l < chunk; l += blockDim.x) {
     double vv[2 * NS];
@@ -811,6 +818,7 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
   const Item it = g.items[item_id];
   const RealInfo ri = g.real[it.r];
   const int chunk = time_chunk_of(g);
+  const int surf = surface_of(g, it);
   const RingLds L = carve(smem, g.n_phi, g.n_sets * g.n_terms, g.n_t, g.time_tables != 0, chunk);
   if (L.tb) prepare_item_tables(g, L, it, ri);
   __syncthreads();
@@ -838,9 +846,9 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
       for (int c = 0; c < 4; ++c) comp[q][s][c] = 0.0f;
 
   for (int i = 0; i < g.n_c; ++i) {
-    if (g.ring_fac[i] == 0.0) continue;  // sin^2 theta = 0: B = 0 exactly
+    if (g.ring_fac[static_cast<int64_t>(surf) * g.n_c + i] == 0.0) continue;  // B = 0 exactly
     bool skip;
-    const int t_lo = prepare_ring(g, L, tau_item, i, it.t_a, ri, &skip);
+    const int t_lo = prepare_ring(g, L, tau_item, i, it.t_a, ri, &skip, surf);
     if (skip || t_lo >= g.n_t) {
       __syncthreads();
       continue;
