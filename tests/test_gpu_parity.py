@@ -331,6 +331,33 @@ def test_masks_bitexact_split_rings(n_c):
     assert rel_per_bin(S, So) <= FP64_REL
 
 
+def test_masks_and_spectra_random_configurations():
+    """Seeded random configurations: 3-13 bubbles with staggered nucleation (beta 6-20),
+    n_cos_thetax 2-39, n_phix 2-139, a random k-hat; every bubble's first colliders and
+    delays bit-exact, S within 1e-10 per bin."""
+    rng = np.random.default_rng(2024)
+    for case in range(16):
+        n = int(rng.integers(3, 14))
+        beta = float(rng.uniform(6, 20))
+        inter = nucleation.exponential_nucleation(beta, n_max=n, seed=int(rng.integers(0, 10**6)))
+        ext = nucleation.face_images(inter, 1.0)
+        n_c, n_phi = int(rng.integers(2, 40)), int(rng.integers(2, 140))
+        print(f"random case {case}: {len(inter)} bubbles, grid {n_c} x {n_phi}", flush=True)
+        bf = flow(inter, ext, n_c, n_phi, [[1.0]], [[5.0]])
+        k = rng.normal(size=3)
+        k /= np.linalg.norm(k)
+        for a in range(len(inter)):
+            f, d = bf.first_colliding_in_frame(a, k)
+            fo, do = oracle.first_colliding(inter, ext, n_c, n_phi, a, k_hat=k)
+            np.testing.assert_array_equal(f, fo)
+            assert d.tobytes() == do.tobytes(), (case, a)
+        w = np.geomspace(0.5, 40.0, 5)
+        t_end = float(inter[:, 0].max()) + 0.5
+        S, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="fp64")
+        So = oracle.spectrum(inter, ext, [k], n_c, n_phi, [[1.0]], [[5.0]], w, t_end=t_end, n_t=60)
+        np.testing.assert_allclose(S, So, rtol=FP64_REL, atol=0)
+
+
 @pytest.mark.parametrize("n_phi", [200, 300, 700])
 def test_masks_bitexact_wide_rings(n_phi):
     """Rings of more than 128 azimuths (sorted in LDS: 256, 512 and 1024 slots)."""
