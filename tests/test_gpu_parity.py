@@ -334,7 +334,8 @@ def test_masks_bitexact_split_rings(n_c):
 def test_masks_and_spectra_random_configurations():
     """Seeded random configurations: 3-13 bubbles with staggered nucleation (beta 6-20),
     n_cos_thetax 2-39, n_phix 2-139, a random k-hat; every bubble's first colliders and
-    delays bit-exact, S within 1e-10 per bin."""
+    delays bit-exact, S within 1e-10 per bin (blocked form 1e-9, mixed form 1e-4 of the
+    FP64 form)."""
     rng = np.random.default_rng(2024)
     for case in range(16):
         n = int(rng.integers(3, 14))
@@ -356,6 +357,10 @@ def test_masks_and_spectra_random_configurations():
         S, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="fp64")
         So = oracle.spectrum(inter, ext, [k], n_c, n_phi, [[1.0]], [[5.0]], w, t_end=t_end, n_t=60)
         np.testing.assert_allclose(S, So, rtol=FP64_REL, atol=0)
+        Sb, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="fp64_blocked")
+        np.testing.assert_allclose(Sb, So, rtol=1e-9, atol=0)
+        Sm, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="mixed")
+        np.testing.assert_allclose(Sm, S, rtol=1e-4, atol=0)
 
 
 @pytest.mark.parametrize("n_phi", [200, 300, 700])
