@@ -178,3 +178,28 @@ def test_grad_128_fd_and_fit_recover_theta_star():
     theta, hist = P.fit(theta0=(1.5, 1.3), steps=200)
     assert abs(theta[0] - THETA_STAR[0]) < 1e-3 and abs(theta[1] - THETA_STAR[1]) < 1e-3, theta
     assert hist[-1][0] < 1e-6 * hist[0][0]
+
+
+def test_vjp_random_configurations():
+    """Seeded random configurations and parameters against the FP64 reference gradient,
+    including a run of 1100 steps (two adjoint chunks of time steps) and integer and
+    non-integer powers."""
+    rng = np.random.default_rng(77)
+    for case in range(6):
+        inter, ext = small_async(int(rng.integers(0, 10**6)), int(rng.integers(3, 8)))
+        n_c, n_phi = int(rng.integers(4, 16)), int(rng.integers(4, 33))
+        n_t = 1100 if case == 0 else int(rng.integers(30, 120))
+        w = np.geomspace(0.5, float(rng.uniform(20.0, 120.0)), 6)
+        k_hat = [rng.normal(size=3)]
+        k_hat[0] /= np.linalg.norm(k_hat[0])
+        p0 = float(rng.integers(3, 7)) if case % 2 else float(rng.uniform(3.0, 6.0))
+        coef, pows = [[float(rng.uniform(0.5, 1.5))]], [[p0]]
+        print(f"random adjoint case {case}: {len(inter)} bubbles, {n_c} x {n_phi}, n_t {n_t}, "
+              f"power {p0:.3f}", flush=True)
+        bf = flow(inter, ext, n_c, n_phi, coef, pows)
+        G = random_cotangent(rng, (1, 2, 1, len(w)))
+        dc, dp = bf.spectrum_vjp(w, k_hat, G, t_end=1.0, n_t=n_t)
+        rc, rp = reference_gradient(inter, ext, n_c, n_phi, coef, pows, w, k_hat, G, t_end=1.0,
+                                    n_t=n_t)
+        assert_rel(dc, rc, REL_ORACLE)
+        assert_rel(dp, rp, REL_ORACLE)
