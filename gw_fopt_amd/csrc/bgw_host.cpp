@@ -337,10 +337,14 @@ struct Device {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
       throw CalcError("no HIP device is available: the bulk-flow kernels need a gfx950 GPU");
     HIPCHECK(hipGetDevice(&id));
+    // `stream` is created last: it marks the device as initialised, so a
+    // failure part-way leaves the rest to be created by the next call.
+    if (!stream2) HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    for (auto& e : ev)
+      if (!e) HIPCHECK(hipEventCreate(&e));
+    for (auto& e : dep)
+      if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
-    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
-    for (auto& e : dep) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   ~Device() {
     for (auto& e : ev)

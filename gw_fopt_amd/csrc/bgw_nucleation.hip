@@ -730,6 +730,17 @@ struct DevMem {
   }
 };
 
+// A timing event, destroyed on every exit path.
+struct TimingEvent {
+  hipEvent_t e = nullptr;
+  TimingEvent() { HIPCHECK(hipEventCreate(&e)); }
+  ~TimingEvent() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  TimingEvent(const TimingEvent&) = delete;
+  TimingEvent& operator=(const TimingEvent&) = delete;
+};
+
 thread_local double t_last_kernel_ms = 0.0;
 
 void need_device() {
@@ -874,11 +885,9 @@ BgwStatus bgw_nucleation_fixed_rate(const BgwFixedRateRealisation* realisations,
     a.counts = d_counts.as<uint64_t>();
     a.status = d_status.as<int32_t>();
 
-    hipEvent_t ev0, ev1;
-    HIPCHECK(hipEventCreate(&ev0));
-    HIPCHECK(hipEventCreate(&ev1));
+    TimingEvent ev0, ev1;
     const int nr = static_cast<int>(n_realisations);
-    HIPCHECK(hipEventRecord(ev0, nullptr));
+    HIPCHECK(hipEventRecord(ev0.e, nullptr));
     switch (K) {
       case 2: launch_fixed_rate<2>(nr, a, nullptr); break;
       case 4: launch_fixed_rate<4>(nr, a, nullptr); break;
@@ -890,13 +899,11 @@ BgwStatus bgw_nucleation_fixed_rate(const BgwFixedRateRealisation* realisations,
       default: launch_fixed_rate<0>(nr, a, nullptr); break;
     }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ev1, nullptr));
-    HIPCHECK(hipEventSynchronize(ev1));
+    HIPCHECK(hipEventRecord(ev1.e, nullptr));
+    HIPCHECK(hipEventSynchronize(ev1.e));
     float ms = 0.f;
-    HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    HIPCHECK(hipEventElapsedTime(&ms, ev0.e, ev1.e));
     t_last_kernel_ms = ms;
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
 
     std::vector<uint64_t> cnt(n_realisations);
     std::vector<int32_t> st(n_realisations);
