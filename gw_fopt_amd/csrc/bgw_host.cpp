@@ -832,6 +832,54 @@ std::vector<int32_t> unit_order(const Engine& E, const Launch& Lc) {
   return order;
 }
 
+// Cache key of the ensemble graph: every field of every argument struct the
+// graph's launches capture (sizes, scalars such as the damping width, and
+// device addresses), so that a replay never runs with stale arguments after a
+// setter or a new k-hat list changed them.  The size checks catch a field
+// added to a struct but not listed here.
+uint64_t key_bits(double x) {
+  uint64_t u;
+  std::memcpy(&u, &x, sizeof(u));
+  return u;
+}
+
+void key_args(std::ostringstream& k, const bgw::GeoArgs& a) {
+  static_assert(sizeof(bgw::GeoArgs) == 144, "GeoArgs changed: list the new field in key_args");
+  k << "|G" << a.cand << ',' << a.inv << ',' << a.real << ',' << a.items << ',' << a.rot << ','
+    << a.identity << ',' << a.sin_t << ',' << a.cos_t << ',' << a.cos_p << ',' << a.sin_p << ','
+    << a.n_c << ',' << a.n_phi << ',' << a.n_phi_pow2 << ',' << a.tau << ',' << a.perm << ','
+    << a.first_out << ',' << a.dtab_out << ',' << a.item0 << ',' << a.rings_per_block << ','
+    << a.n_split;
+}
+
+void key_args(std::ostringstream& k, const bgw::SpecArgs& a) {
+  static_assert(sizeof(bgw::SpecArgs) == 232, "SpecArgs changed: list the new field in key_args");
+  k << "|S" << a.real << ',' << a.items << ',' << a.order << ',' << a.tau << ',' << a.perm << ','
+    << a.gcos << ',' << a.gsin << ',' << a.ring_c << ',' << a.ring_fac << ',' << a.omega << ','
+    << a.coef << ',' << a.pows << ',' << a.n_sets << ',' << a.n_terms << ',' << a.set0 << ','
+    << a.n_sets_out << ',' << a.n_c << ',' << a.n_phi << ',' << a.n_t << ',' << a.n_w << ','
+    << a.cutoff_kind << ',' << key_bits(a.t_cut) << ',' << key_bits(a.inv_t0_sq) << ','
+    << a.has_damping << ',' << key_bits(a.damping) << ',' << key_bits(a.kfac) << ','
+    << a.skip_uncollided << ',' << a.mixed_block << ',' << a.threads << ',' << a.time_tables
+    << ',' << a.time_chunk << ',' << a.surfaces << ',' << a.partial << ',' << a.item0 << ','
+    << a.grad_c << ',' << a.grad_out;
+}
+
+void key_args(std::ostringstream& k, const bgw::RedArgs& a) {
+  static_assert(sizeof(bgw::RedArgs) == 88, "RedArgs changed: list the new field in key_args");
+  k << "|R" << a.partial << ',' << a.group_off << ',' << a.group_r << ',' << a.real << ','
+    << a.omega << ',' << a.n_sets << ',' << a.n_w << ',' << a.c_out << ',' << a.s_out << ','
+    << a.unit_group << ',' << a.unit_off << ',' << a.u0 << ',' << a.item0;
+}
+
+void key_args(std::ostringstream& k, const bgw::EnsArgs& a) {
+  static_assert(sizeof(bgw::EnsArgs) == 432, "EnsArgs changed: list the new field in key_args");
+  key_args(k, a.geo);
+  key_args(k, a.spec);
+  k << "|E" << a.unit_group << ',' << a.group_item_off << ',' << a.group_r << ',' << a.n_units
+    << ',' << a.counter << ',' << a.c_out << ',' << a.s_out;
+}
+
 // Launch arguments of one chunk of the pipelined form.
 struct EnsChunk {
   int item0 = 0, n_items = 0;  // items [item0, item0 + n_items) of the unit-ordered list
@@ -843,8 +891,8 @@ struct EnsChunk {
 // tiles.items_per_chunk == 0: one fused persistent launch per set chunk;
 // otherwise chunks of units, geometry on one stream and phase sums + reduce on
 // the other, the delay rows double-buffered.  Either way the whole sequence
-// is one captured hipGraph, re-captured only when the launch shape, the sizes
-// or a device buffer changes.  replays >= 1 graph launches; ms is their mean.
+// is one captured hipGraph, re-captured only when an argument of one of its
+// launches changes (key_args).  replays >= 1 graph launches; ms is their mean.
 EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
                        const std::vector<int32_t>* units, bool want_outputs, int replays) {
   const Resolution& res = *q.res;
@@ -917,6 +965,7 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
       eb.geo.perm = E.d_perm.as<uint16_t>();
       eb.spec.tau = eb.geo.tau;
       eb.spec.perm = eb.geo.perm;
+      key_args(key, eb);
       body.push_back([eb, &q, n_blocks, st]() { return bgw::launch_ensemble(eb, q.mode, n_blocks, st); });
     }
     key << ':' << n_blocks;
@@ -1005,6 +1054,8 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
       gc.item0 = c.item0;
       gc.tau = tau0 + static_cast<size_t>(c.buf) * rows * n_dir;
       gc.perm = perm0 + static_cast<size_t>(c.buf) * rows * n_dir;
+      key_args(key, gc);
+      key << ',' << c.n_items;
       body.push_back([=]() {
         hipError_t e = hipSuccess;
         if (ci >= 2) e = hipStreamWaitEvent(st, spec_done[c.buf], 0);
@@ -1026,11 +1077,14 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
         sb.time_tables = 0;
         sb.time_tables = bgw::spectrum_time_tables(sb) ? 1 : 0;
         const int mode = q.mode;
+        key_args(key, sb);
         body.push_back([=]() { return bgw::launch_spectrum(sb, c.n_items, mode, s2); });
       }
       bgw::RedArgs rc = ra;
       rc.u0 = c.u0;
       rc.item0 = c.item0;
+      key_args(key, rc);
+      key << ',' << c.n_units;
       body.push_back([=]() {
         hipError_t e = bgw::launch_reduce(rc, c.n_units, s2);
         if (e == hipSuccess) e = hipEventRecord(spec_done[c.buf], s2);
@@ -1060,7 +1114,7 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
       << static_cast<const void*>(E.d_group_off.p) << ':' << static_cast<const void*>(E.d_c.p)
       << ':' << static_cast<const void*>(E.d_s.p) << ':' << static_cast<const void*>(E.d_cand.p)
       << ':' << static_cast<const void*>(E.d_rot.p) << ':' << static_cast<const void*>(E.d_kmean.p)
-      << ':' << static_cast<const void*>(E.d_emean.p);
+      << ':' << static_cast<const void*>(E.d_emean.p) << ':' << static_cast<const void*>(E.d_estd.p);
   if (!E.graph || E.graph_key != key.str()) {
     if (E.graph) {
       HIPCHECK(hipGraphExecDestroy(E.graph));

@@ -150,3 +150,36 @@ def test_autotune_picks_a_valid_shape(ens):
     assert tuned["S"].tobytes() == explicit["S"].tobytes()
     t = E.last_timings()
     assert t["n_items"] == int(sum(len(b) for b in inter)) * len(k)
+
+
+@pytest.mark.parametrize("chunk", [0, 30])
+def test_setters_and_new_directions_between_calls(ens, chunk):
+    """A captured graph is replayed only while every launch argument is
+    unchanged: a smaller grid (no buffer grows), a damping width, more terms
+    and another k-hat list of the same length each give the spectrum of a
+    fresh ensemble with those settings, bit for bit."""
+    inter, ext, t_end, w, k = ens
+    tl = {"items_per_chunk": chunk}
+
+    def fresh(n_c, n_phi, coef, pows, damping, kk):
+        F = BubbleEnsemble(inter, ext)
+        F.set_resolution(n_c, n_phi)
+        F.set_gradient_scaling_params(coef, pows, damping)
+        return F.compute_spectrum(w, kk, t_end=t_end, n_t=N_T, precision="fp64", tiles=tl)["S"]
+
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params([[1.0]], [[5.0]])
+    E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64", tiles=tl)
+    steps = [
+        (N_C - 4, N_PHI - 8, [[1.0]], [[5.0]], None, k),
+        (N_C - 4, N_PHI - 8, [[1.0]], [[5.0]], 0.3, k),
+        (N_C - 4, N_PHI - 8, [[1.0, 0.5]], [[5.0, 4.0]], 0.3, k),
+        (N_C - 4, N_PHI - 8, [[1.0, 0.5]], [[5.0, 4.0]], 0.3, k[::-1].copy()),
+    ]
+    for n_c, n_phi, coef, pows, damping, kk in steps:
+        E.set_resolution(n_c, n_phi)
+        E.set_gradient_scaling_params(coef, pows, damping)
+        got = E.compute_spectrum(w, kk, t_end=t_end, n_t=N_T, precision="fp64", tiles=tl)["S"]
+        want = fresh(n_c, n_phi, coef, pows, damping, kk)
+        assert got.tobytes() == want.tobytes(), (n_c, n_phi, coef, damping)
