@@ -59,3 +59,36 @@ def test_tensor_selected_and_integrand():
         bf.compute_c_tensor(w, t_end=1.5, n_t=70, selected_bubbles=[7])
     with pytest.raises(ValueError, match="n_t must be >= 2"):
         bf.compute_c_tensor(w, t_end=1.5, n_t=1)
+
+
+def test_tensor_random_configurations():
+    """Seeded random configurations: 3-13 bubbles with staggered nucleation (beta 6-20),
+    n_cos_thetax 2-39, n_phix 3-139, two parameter sets (an integer and a non-integer
+    power), with and without damping.  Each component is judged against its largest entry,
+    or 1e-3 of the largest entry of all six where a component is at rounding level (a
+    coarse azimuth grid): 1e-10 for the FP64 form, 1e-9 for the blocked form."""
+    rng = np.random.default_rng(77)
+    for case in range(8):
+        n = int(rng.integers(3, 14))
+        beta = float(rng.uniform(6, 20))
+        inter = nucleation.exponential_nucleation(beta, n_max=n, seed=int(rng.integers(0, 10**6)))
+        ext = nucleation.face_images(inter, 1.0)
+        n_c, n_phi = int(rng.integers(2, 40)), int(rng.integers(3, 140))
+        coef = [[float(rng.uniform(0.5, 1.5))], [float(rng.uniform(0.5, 1.5))]]
+        pows = [[5.0], [float(rng.uniform(2.5, 6.0))]]
+        damping = None if case % 2 == 0 else float(rng.uniform(0.2, 1.0))
+        print(f"tensor random case {case}: {len(inter)} bubbles, grid {n_c} x {n_phi}, "
+              f"damping {damping}", flush=True)
+        bf = _flow(inter, ext, n_c, n_phi, coef, pows)
+        if damping is not None:
+            bf.set_gradient_scaling_params(coef, pows, damping)
+        w = np.geomspace(0.5, 40.0, 5)
+        t_end = float(inter[:, 0].max()) + 0.5
+        To = oracle.compute_c_tensor(inter, ext, n_c, n_phi, coef, pows, w, t_end=t_end, n_t=60,
+                                     damping=damping)
+        floor = 1e-3 * np.max(np.abs(To))
+        for precision, bar in (("fp64", 1e-10), ("fp64_blocked", 1e-9)):
+            T = bf.compute_c_tensor(w, t_end=t_end, n_t=60, precision=precision)
+            for c in range(6):
+                err = np.max(np.abs(T[c] - To[c])) / max(np.max(np.abs(To[c])), floor)
+                assert err <= bar, (case, precision, c, err)
