@@ -160,6 +160,7 @@ int ensemble_blocks_per_cu(const EnsArgs& a, int mode) {
   if (!k) return 0;
   int nb = 0;
   const size_t lds = ensemble_lds_bytes(a, mode);
+  std::lock_guard<std::mutex> lock(lds_limit_mutex());
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                           hipFuncAttributeMaxDynamicSharedMemorySize,
                           static_cast<int>(lds)) != hipSuccess)
@@ -170,9 +171,8 @@ int ensemble_blocks_per_cu(const EnsArgs& a, int mode) {
   return nb;
 }
 
-// ensemble_blocks_per_cu must have been called for this launch shape first (it
-// sets the kernel's dynamic-LDS limit, which is not a stream operation and so
-// stays out of a captured graph).
+// Sets the kernel's dynamic-LDS limit again, under the lock, right before the
+// launch (it is not a stream operation and stays out of a captured graph).
 hipError_t launch_ensemble(const EnsArgs& a, int mode, int n_blocks, hipStream_t s) {
   int threads = 0;
   EnsKernel k = ensemble_instance(a, mode, &threads);
@@ -180,6 +180,10 @@ hipError_t launch_ensemble(const EnsArgs& a, int mode, int n_blocks, hipStream_t
   const size_t lds = ensemble_lds_bytes(a, mode);
   hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(lds_limit_mutex());
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(n_blocks), dim3(threads), lds, s, a);
   return hipGetLastError();

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "bgw_common.h"
 
 namespace bgw {
@@ -124,6 +126,12 @@ inline int resolve_threads(int threads, int n_w, int mode) {
   int t = ((n_w + per_lane - 1) / per_lane + 63) / 64 * 64;
   return t < 64 ? 64 : (t > 256 ? 256 : t);
 }
+
+// The dynamic-LDS limit (hipFuncSetAttribute) is per-function process state:
+// each launcher sets it, queries occupancy under it and launches while holding
+// this lock, so host threads driving separate handles cannot interleave them.
+// Launchers take it themselves; callers must not hold it.
+std::mutex& lds_limit_mutex();
 
 hipError_t launch_geometry(const GeoArgs& a, int n_items, hipStream_t s);
 hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mixed, hipStream_t s);

@@ -38,6 +38,9 @@
 //   (deterministic) and form S = 8 pi w^3 (|C+|^2 + |Cx|^2).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "bgw_stages.h"
 
 namespace bgw {
@@ -497,18 +500,24 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SpecArgs g) {
 // Workgroups per CU the geometry kernel keeps resident (64 VGPRs, 20 KB of LDS).
 constexpr int kGeoResident = 8;
 
+std::mutex& lds_limit_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 int device_cus() {
-  static int cus[64] = {0};
+  // per-device cache; handles on several host threads may fill it at once
+  static std::atomic<int> cus[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int n = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         n <= 0)
       n = 256;
-    cus[dev] = n;
+    cus[dev].store(n, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return n;
 }
 
 hipError_t launch_geometry(const GeoArgs& a_in, int n_items, hipStream_t s) {
@@ -578,6 +587,7 @@ hipError_t launch_spec_t(const SpecArgs& a_in, int n_items, int threads, size_t 
                          hipStream_t s) {
   auto k = spectrum_kernel<NS, OPL, MODE, RB>;
   const void* kp = reinterpret_cast<const void*>(k);
+  std::lock_guard<std::mutex> lock(lds_limit_mutex());
   SpecArgs a = a_in;
   a.time_chunk = kTimeChunkMin;
   size_t lds = spectrum_lds_bytes(a, MODE);
@@ -647,6 +657,7 @@ namespace {
 template <int NS, int RB>
 hipError_t launch_adj_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
   auto k = adjoint_kernel<NS, RB>;
+  std::lock_guard<std::mutex> lock(lds_limit_mutex());
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
@@ -673,6 +684,7 @@ namespace {
 template <int NS, int OPL>
 hipError_t launch_int_t(const SpecArgs& a, int n_items, int threads, size_t lds, hipStream_t s) {
   auto k = integrand_kernel<NS, OPL>;
+  std::lock_guard<std::mutex> lock(lds_limit_mutex());
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
