@@ -64,6 +64,9 @@ def parse():
                         "autotuner picks for it (for rocprofv3)")
     p.add_argument("--profile-grad", action="store_true",
                    help="only run --steps grad_128 gradients (forward + adjoint, for rocprofv3)")
+    p.add_argument("--profile-tensor", default=None, metavar="FORM",
+                   help="only run --steps compute_c_tensor calls (the tensor form C_ij) in this "
+                        "form at plain_256's bubbles, grid, times and omegas (for rocprofv3)")
     p.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic counters")
     p.add_argument("--modes", default="mixed,fp64_blocked,fp64")
     p.add_argument("--profile-mode", default=None,
@@ -390,6 +393,24 @@ def main():
         for _ in range(args.steps):
             P.loss_and_grad((1.8, 1.1))
         print(json.dumps({"profile": "grad_128", "gradients": args.steps}), flush=True)
+        return
+    if args.profile_tensor:
+        from gw_fopt_amd import many_bubbles as mb
+        tw = configs.plain_256(n_k=1)
+        bf = mb.NewGeneralizedBulkFlow(mb.LatticeBubbles(mb.EmptyLattice(), tw.interior, tw.exterior))
+        bf.set_resolution(tw.n_cos_thetax, tw.n_phix)
+        bf.set_gradient_scaling_params(tw.coefficients_sets, tw.powers_sets)
+        form = args.profile_tensor
+        for _ in range(args.warmup):
+            bf.compute_c_tensor(tw.w_arr, t_end=tw.t_end, n_t=tw.n_t, precision=form)
+        walls = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            # returns host arrays, so the call has finished on the device
+            bf.compute_c_tensor(tw.w_arr, t_end=tw.t_end, n_t=tw.n_t, precision=form)
+            walls.append((time.perf_counter() - t0) * 1e3)
+        print(json.dumps({"profile": "tensor plain_256", "form": form, "calls": args.steps,
+                          "wall_ms_per_call": walls}), flush=True)
         return
     wl = configs.plain_256(n_k=args.n_k)
     if args.profile_mode:
