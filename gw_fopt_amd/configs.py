@@ -103,33 +103,42 @@ class Ensemble:
 
 
 def _ensemble_member(args):
-    beta, n_target, box, seed = args
-    b = nucleation.exponential_nucleation(beta, box=box, n_max=n_target, seed=seed)
-    return b, nucleation.face_images(b, box), nucleation.percolation_time(b, box=box, n_probe=8192)
+    """One realisation: nucleation in a box of side ``box`` stopped at ``n_target``
+    bubbles.  A draw that percolates with fewer than ``n_floor`` bubbles is redrawn
+    in a box grown for the missing count (next seed of the realisation's stream),
+    so every realisation holds at least ``n_floor`` bubbles."""
+    beta, n_target, box, seed, n_floor = args
+    for attempt in range(16):
+        b = nucleation.exponential_nucleation(beta, box=box, n_max=n_target,
+                                              seed=seed + 7919 * attempt)
+        if len(b) >= n_floor:
+            break
+        box *= float((1.25 * n_floor / max(len(b), 1)) ** (1.0 / 3.0))
+    else:
+        raise RuntimeError(f"realisation seed {seed}: fewer than {n_floor} bubbles")
+    return b, nucleation.face_images(b, box), nucleation.percolation_time(b, box=box, n_probe=8192), box
 
 
-def ensemble_512(n_real: int = 512, seed: int = 4, n_k: int = 32, workers: int | None = None,
-                 n_min: float = 20.0, n_max: float = 400.0) -> Ensemble:
-    """Realisation r: beta L_r ~ U[10, 30]; a target count n_r log-uniform in
-    [n_min, n_max] (mean ~128 for 20..400); box side for ~1.5 n_r bubbles at
-    percolation (density ~0.68 beta^3 / (8 pi)), nucleation stopped at n_r
-    bubbles; t_end = the realisation's percolation time.  Realisations are
-    generated in parallel worker processes (fork); call this before any GPU
-    work in the process."""
-    import multiprocessing
-    import os
-    from concurrent.futures import ProcessPoolExecutor
-
+def _ensemble_jobs(n_real: int, seed: int, n_min: float, n_max: float):
+    """(beta, target count, box side, seed, floor) of each realisation, drawn in order."""
     rng = np.random.default_rng(seed)
     jobs = []
     for r in range(n_real):
         beta = rng.uniform(10.0, 30.0)
         n_target = int(round(float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))))
         box = float((n_target * 8.0 * np.pi / (0.68 * beta ** 3)) ** (1.0 / 3.0))
-        jobs.append((beta, n_target, box, seed * 100003 + r))
+        jobs.append((beta, n_target, box, seed * 100003 + r, int(round(n_min))))
+    return jobs
+
+
+def _ensemble(name, jobs, n_k, workers):
+    import multiprocessing
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+
     if workers is None:
         workers = min(16, os.cpu_count() or 1)
-    if workers > 1 and n_real > 1:
+    if workers > 1 and len(jobs) > 1:
         ctx = multiprocessing.get_context("fork")
         with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
             res = list(ex.map(_ensemble_member, jobs, chunksize=4))
@@ -138,6 +147,26 @@ def ensemble_512(n_real: int = 512, seed: int = 4, n_k: int = 32, workers: int |
     interiors = [r[0] for r in res]
     exteriors = [r[1] for r in res]
     t_end = np.array([r[2] for r in res])
-    boxes = np.array([j[2] for j in jobs])
-    return Ensemble("ensemble_512", interiors, exteriors, boxes, t_end, 32, 64, 500,
+    boxes = np.array([r[3] for r in res])
+    return Ensemble(name, interiors, exteriors, boxes, t_end, 32, 64, 500,
                     np.geomspace(0.5, 300.0, 128), fibonacci_directions(n_k))
+
+
+def ensemble_512(n_real: int = 512, seed: int = 4, n_k: int = 32, workers: int | None = None,
+                 n_min: float = 20.0, n_max: float = 400.0) -> Ensemble:
+    """Realisation r: beta L_r ~ U[10, 30]; a target count n_r log-uniform in
+    [n_min, n_max] (mean ~128 for 20..400); box side for ~1.5 n_r bubbles at
+    percolation (density ~0.68 beta^3 / (8 pi)), nucleation stopped at n_r
+    bubbles, and redrawn in a larger box if it percolates below n_min bubbles;
+    t_end = the realisation's percolation time.  Realisations are generated in
+    parallel worker processes (fork); call this before any GPU work in the
+    process."""
+    return _ensemble("ensemble_512", _ensemble_jobs(n_real, seed, n_min, n_max), n_k, workers)
+
+
+def ensemble_512_members(indices, seed: int = 4, n_k: int = 32, n_real: int = 512,
+                         n_min: float = 20.0, n_max: float = 400.0) -> Ensemble:
+    """Realisations ``indices`` of ``ensemble_512`` (the same bubbles, boxes and time
+    grids), generated alone."""
+    jobs = _ensemble_jobs(n_real, seed, n_min, n_max)
+    return _ensemble("ensemble_512", [jobs[i] for i in indices], n_k, workers=1)

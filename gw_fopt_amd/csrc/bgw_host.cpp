@@ -950,6 +950,9 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
       eb.spec.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
       eb.spec.time_tables = 0;
       eb.spec.time_tables = bgw::spectrum_time_tables(eb.spec) ? 1 : 0;
+      // the chunk length of the per-item phase-sum kernel, so the fused form adds
+      // the same terms in the same order as the single-configuration path
+      eb.spec.time_chunk = bgw::spectrum_time_chunk(eb.spec, q.mode);
       if (bgw::ensemble_lds_bytes(eb, q.mode) + 24 * 1024 > 160 * 1024)
         throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
       const int per_cu = bgw::ensemble_blocks_per_cu(eb, q.mode);  // sets the LDS limit too

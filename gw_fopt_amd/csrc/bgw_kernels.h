@@ -150,5 +150,8 @@ hipError_t launch_accumulate(const double* partial, int n_chunk_items, int64_t n
                              double* acc, hipStream_t s);
 size_t spectrum_lds_bytes(const SpecArgs& a, int mode);
 bool spectrum_time_tables(const SpecArgs& a);
+// Time steps per chunk (SpecArgs::time_chunk) that launch_spectrum takes for
+// the phase-sum kernel of this shape.
+int spectrum_time_chunk(const SpecArgs& a, int mode);
 
 }  // namespace bgw

@@ -578,29 +578,31 @@ int resident_blocks(const void* k, int threads, size_t lds) {
 }
 
 // The chunk length: 512 steps (half the anchors and barriers of 256) when the
-// run is longer than 256 steps and the larger chunk costs no resident
-// workgroups; 256 otherwise.  Measured: plain_256 blocked 27.5 -> 26.8 ms with
-// 512; ensemble_512 (64-lane workgroups, LDS-bound occupancy) 6-17 % slower
-// with 512, so it keeps 256.
+// run is longer than 256 steps and the larger chunk costs the phase-sum kernel
+// of this shape no resident workgroups; 256 otherwise.  Measured: plain_256
+// blocked 27.5 -> 26.8 ms with 512; ensemble_512 (64-lane workgroups,
+// LDS-bound occupancy) 6-17 % slower with 512, so it keeps 256.  The caller
+// holds the LDS-limit lock.  The fused ensemble kernel takes the same choice
+// (spectrum_time_chunk), so every form adds the same terms in the same order.
+void choose_time_chunk(const void* kp, int threads, SpecArgs& a) {
+  a.time_chunk = kTimeChunkMin;
+  if (a.n_t <= kTimeChunkMin) return;
+  SpecArgs b = a;
+  b.time_chunk = kTimeChunkMax;
+  const size_t lds = spectrum_lds_bytes(a, 0), lds_b = spectrum_lds_bytes(b, 0);
+  if (lds_b > 160 * 1024) return;
+  const int nb = resident_blocks(kp, threads, lds_b);
+  if (nb > 0 && nb >= resident_blocks(kp, threads, lds)) a.time_chunk = kTimeChunkMax;
+}
+
 template <int NS, int OPL, int MODE, int RB>
-hipError_t launch_spec_t(const SpecArgs& a_in, int n_items, int threads, size_t /*lds*/,
-                         hipStream_t s) {
+hipError_t launch_spec_t(const SpecArgs& a_in, int n_items, int threads, hipStream_t s) {
   auto k = spectrum_kernel<NS, OPL, MODE, RB>;
   const void* kp = reinterpret_cast<const void*>(k);
   std::lock_guard<std::mutex> lock(lds_limit_mutex());
   SpecArgs a = a_in;
-  a.time_chunk = kTimeChunkMin;
-  size_t lds = spectrum_lds_bytes(a, MODE);
-  if (a.n_t > kTimeChunkMin) {
-    SpecArgs b = a;
-    b.time_chunk = kTimeChunkMax;
-    const size_t lds_b = spectrum_lds_bytes(b, MODE);
-    if (lds_b <= 160 * 1024 && resident_blocks(kp, threads, lds_b) >= resident_blocks(kp, threads, lds) &&
-        resident_blocks(kp, threads, lds_b) > 0) {
-      a = b;
-      lds = lds_b;
-    }
-  }
+  choose_time_chunk(kp, threads, a);
+  const size_t lds = spectrum_lds_bytes(a, MODE);
   hipError_t e = hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
   if (e != hipSuccess) return e;
@@ -608,31 +610,44 @@ hipError_t launch_spec_t(const SpecArgs& a_in, int n_items, int threads, size_t 
   return hipGetLastError();
 }
 
+// Launches the phase-sum kernel of the shape (launch != 0), or only returns
+// its chunk length in *chunk (launch == 0, nothing is queued).
+template <int NS, int OPL, int MODE, int RB>
+hipError_t spec_t(const SpecArgs& a, int n_items, int threads, hipStream_t s, bool launch,
+                  int* chunk) {
+  if (launch) return launch_spec_t<NS, OPL, MODE, RB>(a, n_items, threads, s);
+  const void* kp = reinterpret_cast<const void*>(spectrum_kernel<NS, OPL, MODE, RB>);
+  std::lock_guard<std::mutex> lock(lds_limit_mutex());
+  SpecArgs b = a;
+  choose_time_chunk(kp, threads, b);
+  *chunk = b.time_chunk;
+  return hipSuccess;
+}
+
 template <int NS, int MODE, int RB>
-hipError_t launch_spec_opl(const SpecArgs& a, int n_items, int opl, int threads, size_t lds,
-                           hipStream_t s) {
+hipError_t spec_opl(const SpecArgs& a, int n_items, int opl, int threads, hipStream_t s,
+                    bool launch, int* chunk) {
   switch (opl) {
-    case 1: return launch_spec_t<NS, 1, MODE, RB>(a, n_items, threads, lds, s);
-    case 2: return launch_spec_t<NS, 2, MODE, RB>(a, n_items, threads, lds, s);
-    default: return launch_spec_t<NS, 4, MODE, RB>(a, n_items, threads, lds, s);
+    case 1: return spec_t<NS, 1, MODE, RB>(a, n_items, threads, s, launch, chunk);
+    case 2: return spec_t<NS, 2, MODE, RB>(a, n_items, threads, s, launch, chunk);
+    default: return spec_t<NS, 4, MODE, RB>(a, n_items, threads, s, launch, chunk);
   }
 }
 
 template <int NS>
-hipError_t launch_spec_ns(const SpecArgs& a, int n_items, int mode, int opl, int threads,
-                          size_t lds, hipStream_t s) {
-  if (mode == 0) return launch_spec_opl<NS, 0, 16>(a, n_items, opl, threads, lds, s);
+hipError_t spec_ns(const SpecArgs& a, int n_items, int mode, int opl, int threads, hipStream_t s,
+                   bool launch, int* chunk) {
+  if (mode == 0) return spec_opl<NS, 0, 16>(a, n_items, opl, threads, s, launch, chunk);
   const bool rb8 = resolve_block_steps(a.mixed_block, mode) == 8;
   if (mode == 1)
-    return rb8 ? launch_spec_opl<NS, 1, 8>(a, n_items, opl, threads, lds, s)
-               : launch_spec_opl<NS, 1, 16>(a, n_items, opl, threads, lds, s);
-  return rb8 ? launch_spec_opl<NS, 2, 8>(a, n_items, opl, threads, lds, s)
-             : launch_spec_opl<NS, 2, 16>(a, n_items, opl, threads, lds, s);
+    return rb8 ? spec_opl<NS, 1, 8>(a, n_items, opl, threads, s, launch, chunk)
+               : spec_opl<NS, 1, 16>(a, n_items, opl, threads, s, launch, chunk);
+  return rb8 ? spec_opl<NS, 2, 8>(a, n_items, opl, threads, s, launch, chunk)
+             : spec_opl<NS, 2, 16>(a, n_items, opl, threads, s, launch, chunk);
 }
-}  // namespace
 
-hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t s) {
-  if (n_items == 0) return hipSuccess;
+hipError_t spec_dispatch(const SpecArgs& a, int n_items, int mode, hipStream_t s, bool launch,
+                         int* chunk) {
   if (mode < 0 || mode > 2) return hipErrorInvalidValue;
   // Lanes own omegas: threads = 64..256 (a multiple of 64), up to 4 omegas
   // per lane.  a.threads / a.mixed_block (0 = default) are the tunable choices.
@@ -641,16 +656,27 @@ hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t
   const int opl_needed = (a.n_w + threads - 1) / threads;
   if (opl_needed > 4) return hipErrorInvalidValue;
   const int opl = opl_needed <= 1 ? 1 : (opl_needed <= 2 ? 2 : 4);
-  const size_t lds = spectrum_lds_bytes(a, mode);
   switch (a.n_sets) {
-    case 1: return launch_spec_ns<1>(a, n_items, mode, opl, threads, lds, s);
-    case 2: return launch_spec_ns<2>(a, n_items, mode, opl, threads, lds, s);
-    case 3: return launch_spec_ns<3>(a, n_items, mode, opl, threads, lds, s);
-    case 4: return launch_spec_ns<4>(a, n_items, mode, opl, threads, lds, s);
-    case 5: return launch_spec_ns<5>(a, n_items, mode, opl, threads, lds, s);
-    case 6: return launch_spec_ns<6>(a, n_items, mode, opl, threads, lds, s);
+    case 1: return spec_ns<1>(a, n_items, mode, opl, threads, s, launch, chunk);
+    case 2: return spec_ns<2>(a, n_items, mode, opl, threads, s, launch, chunk);
+    case 3: return spec_ns<3>(a, n_items, mode, opl, threads, s, launch, chunk);
+    case 4: return spec_ns<4>(a, n_items, mode, opl, threads, s, launch, chunk);
+    case 5: return spec_ns<5>(a, n_items, mode, opl, threads, s, launch, chunk);
+    case 6: return spec_ns<6>(a, n_items, mode, opl, threads, s, launch, chunk);
     default: return hipErrorInvalidValue;
   }
+}
+}  // namespace
+
+int spectrum_time_chunk(const SpecArgs& a, int mode) {
+  int chunk = kTimeChunkMin;
+  if (spec_dispatch(a, 0, mode, nullptr, false, &chunk) != hipSuccess) return kTimeChunkMin;
+  return chunk;
+}
+
+hipError_t launch_spectrum(const SpecArgs& a, int n_items, int mode, hipStream_t s) {
+  if (n_items == 0) return hipSuccess;
+  return spec_dispatch(a, n_items, mode, s, true, nullptr);
 }
 
 namespace {

@@ -46,7 +46,9 @@
 
 // Test-only switch: a nonzero ORACLE_PLANT compiles a deliberately wrong
 // variant (1: sign of the time phase, 2: cos-theta step 2/n instead of
-// 2/(n-1), 3: exact ties accepted as first collisions),
+// 2/(n-1), 3: exact ties accepted as first collisions, 4: the k-hat frame
+// rotated by the transpose of R, 5: the deltas left unrotated in the k-hat
+// frame),
 // used by the tests to show that the oracle's checks catch such mistakes.
 #ifndef ORACLE_PLANT
 #define ORACLE_PLANT 0
@@ -232,7 +234,13 @@ inline void rot3(const double* R, const double* v, double* out) {
 // LatticeBubbles::transform with a pure rotation (lattice_bubbles.rs:144-197,
 // bubbles.rs:47-88): rotate positions and delta spatial parts; delta_squared
 // stays as computed from the unrotated bubbles.
-void rotate_lattice(Lattice& L, const double* R) {
+void rotate_lattice(Lattice& L, const double* Rk) {
+#if ORACLE_PLANT == 4
+  // planted mistake: the inverse rotation
+  const double R[9] = {Rk[0], Rk[3], Rk[6], Rk[1], Rk[4], Rk[7], Rk[2], Rk[5], Rk[8]};
+#else
+  const double* R = Rk;
+#endif
   double tmp[3];
   for (size_t b = 0; b < L.n_int; ++b) {
     rot3(R, &L.interior[b * 4 + 1], tmp);
@@ -242,6 +250,9 @@ void rotate_lattice(Lattice& L, const double* R) {
     rot3(R, &L.exterior[b * 4 + 1], tmp);
     for (int k = 0; k < 3; ++k) L.exterior[b * 4 + 1 + k] = tmp[k];
   }
+#if ORACLE_PLANT == 5
+  return;  // planted mistake: the deltas keep the unrotated frame
+#endif
   for (size_t i = 0; i < L.n_int * L.n_tot; ++i) {
     rot3(R, &L.delta[i * 4 + 1], tmp);
     for (int k = 0; k < 3; ++k) L.delta[i * 4 + 1 + k] = tmp[k];

@@ -59,8 +59,9 @@ def first_collision(bubbles, a, n_int, x, tol=1e-10):
     return first, np.where(found, best, 0.0)
 
 
-def c_integral(interior, exterior, n_c, n_phi, coefficients, powers, w, t_begin, t_end, n_t):
-    """(2, n_sets, n_w) complex; bubbles (t, x, y, z); EmptyLattice semantics."""
+def c_integrand(interior, exterior, n_c, n_phi, coefficients, powers, w, t_begin, t_end, n_t):
+    """(2, n_sets, n_w, n_t) complex: the terms of C at each time step (their quadrature
+    weights included, zero for t <= t_n), summed over the bubbles."""
     interior = np.asarray(interior, float).reshape(-1, 4)
     exterior = np.zeros((0, 4)) if exterior is None else np.asarray(exterior, float).reshape(-1, 4)
     bubbles = np.vstack([interior, exterior])
@@ -78,7 +79,7 @@ def c_integral(interior, exterior, n_c, n_phi, coefficients, powers, w, t_begin,
     wp = np.full(n_phi, 2.0 * np.pi / n_phi)
     wp[[0, -1]] *= 0.5
     g = np.stack([np.cos(2 * phi), np.sin(2 * phi)])          # (2, n_phi)
-    out = np.zeros((2, len(coefficients), len(w)), dtype=complex)
+    out = np.zeros((2, len(coefficients), len(w), n_t), dtype=complex)
     for a in range(n_int):
         first, delay = first_collision(bubbles, a, n_int, x)
         ta, za = interior[a, 0], interior[a, 3]
@@ -96,5 +97,16 @@ def c_integral(interior, exterior, n_c, n_phi, coefficients, powers, w, t_begin,
             ang = np.exp(-1j * w[:, None, None] * dta[None, :, None] * cos_t[None, None, :])
             A = np.einsum("gti,wti->gwt", B * wc[None, None, :], ang)   # (2, n_w, n_t)
             ph = np.exp(1j * w[:, None] * (t[None, :] - za)) * (wt * dt * (t >= ta))[None, :]
-            out[:, s, :] += np.sum(A * ph[None], axis=-1) / (6 * np.pi)
+            out[:, s] += A * ph[None] / (6 * np.pi)
     return out
+
+
+def c_integral(interior, exterior, n_c, n_phi, coefficients, powers, w, t_begin, t_end, n_t,
+               cutoff=None):
+    """(2, n_sets, n_w) complex; bubbles (t, x, y, z); EmptyLattice semantics.  ``cutoff``:
+    None, or a function of the time grid giving the weight of each step."""
+    I = c_integrand(interior, exterior, n_c, n_phi, coefficients, powers, w, t_begin, t_end, n_t)
+    if cutoff is not None:
+        t = t_begin + (t_end - t_begin) / (n_t - 1) * np.arange(n_t)
+        I = I * np.asarray(cutoff(t), float)
+    return np.sum(I, axis=-1)

@@ -57,7 +57,7 @@ def test_reference_bench_configuration():
     five simultaneous bubbles, 50 x 100 grid (rings of 100 azimuths in 128 sort slots),
     sets [[0], [1]] / [[3], [3]], 120 omegas geomspace(1e-2, 1e2), t in [0, 15], 800 steps.
     First colliders bit-exact for every bubble; C within 1e-10 of the largest entry of
-    each (polarisation, set) row; S per bin within 1e-10 (blocked form 1e-9)."""
+    each (polarisation, set) row; S per bin within 1e-10 in both FP64 forms."""
     inter = np.array([[0.0, 0.0, 10.0, 0.0], [0.0, 0.0, 0.0, 0.0], [0.0, 4.0, 0.0, 4.0],
                       [0.0, 4.0, 2.0, 4.0], [0.0, 0.0, 2.0, 1.0]])
     coef, pows = [[0.0], [1.0]], [[3.0], [3.0]]
@@ -77,7 +77,7 @@ def test_reference_bench_configuration():
             assert rel_to_max(C[pm, s], Co[pm, s]) <= FP64_REL, (pm, s)
     k = mb.fibonacci_directions(2)
     So = oracle.spectrum(inter, None, k, 50, 100, coef, pows, w, t_begin=0.0, t_end=15.0, n_t=800)
-    for precision, bar in (("fp64", FP64_REL), ("fp64_blocked", 1e-9)):
+    for precision, bar in (("fp64", FP64_REL), ("fp64_blocked", FP64_REL)):
         S, _ = bf.compute_spectrum(w, k, t_begin=0.0, t_end=15.0, n_t=800, precision=precision)
         assert rel_per_bin(S, So) <= bar, precision
 
@@ -220,7 +220,7 @@ def test_spectrum_multi_chunk_blocked_and_mixed():
     S64, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100)
     assert rel_per_bin(S64, So) <= FP64_REL
     Sb, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="fp64_blocked")
-    assert rel_per_bin(Sb, So) <= 1e-9
+    assert rel_per_bin(Sb, So) <= FP64_REL
     S32, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="mixed")
     rel = np.abs(S32 - S64) / np.abs(S64)
     # 5.5x more time steps than plain_8: the cancellation floor of the float
@@ -334,7 +334,7 @@ def test_masks_bitexact_split_rings(n_c):
 def test_masks_and_spectra_random_configurations():
     """Seeded random configurations: 3-13 bubbles with staggered nucleation (beta 6-20),
     n_cos_thetax 2-39, n_phix 2-139, a random k-hat; every bubble's first colliders and
-    delays bit-exact, S within 1e-10 per bin (blocked form 1e-9, mixed form 1e-4 of the
+    delays bit-exact, S within 1e-10 per bin in both FP64 forms (mixed form 1e-4 of the
     FP64 form)."""
     rng = np.random.default_rng(2024)
     for case in range(16):
@@ -358,7 +358,7 @@ def test_masks_and_spectra_random_configurations():
         So = oracle.spectrum(inter, ext, [k], n_c, n_phi, [[1.0]], [[5.0]], w, t_end=t_end, n_t=60)
         np.testing.assert_allclose(S, So, rtol=FP64_REL, atol=0)
         Sb, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="fp64_blocked")
-        np.testing.assert_allclose(Sb, So, rtol=1e-9, atol=0)
+        np.testing.assert_allclose(Sb, So, rtol=FP64_REL, atol=0)
         Sm, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="mixed")
         np.testing.assert_allclose(Sm, S, rtol=1e-4, atol=0)
 
@@ -380,7 +380,7 @@ def test_masks_bitexact_wide_rings(n_phi):
     assert rel_per_bin(S, So) <= FP64_REL
 
 
-@pytest.mark.parametrize("precision,bar", [("fp64", FP64_REL), ("fp64_blocked", 1e-9)])
+@pytest.mark.parametrize("precision,bar", [("fp64", FP64_REL), ("fp64_blocked", FP64_REL)])
 def test_spectrum_many_omegas(precision, bar):
     """600 omegas: three omegas per lane of a 256-lane workgroup."""
     wl = configs.plain_8()

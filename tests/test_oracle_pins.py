@@ -224,6 +224,103 @@ def test_rotation_maps_k_to_z():
         assert abs(np.linalg.det(R) - 1) < 1e-14
 
 
+def rotate(b, R):
+    """Bubbles (t, x, y, z) with positions rotated by R."""
+    b = np.asarray(b, float).reshape(-1, 4)
+    return np.column_stack([b[:, 0], b[:, 1:] @ R.T])
+
+
+@pytest.mark.parametrize("seed", [4, 16])
+def test_k_hat_frame_matches_numpy_on_rotated_bubbles(seed):
+    """The oracle's k-hat frame (positions and deltas rotated, delta^2 kept,
+    lattice_bubbles.rs:144-197) against the restatement run on bubbles rotated
+    beforehand (delta^2 recomputed): masks equal, delays and C to rounding."""
+    inter, ext = async_bubbles(seed, n=6)
+    k = np.random.default_rng(seed).normal(size=3)
+    k /= np.linalg.norm(k)
+    R = oracle.rotation_to_z(k)
+    ri, re = rotate(inter, R), rotate(ext, R)
+    allb = np.vstack([ri, re])
+    _, _, dirs = ind.directions(8, 12)
+    for a in range(len(inter)):
+        first, dtab = oracle.first_colliding(inter, ext, 8, 12, a, k_hat=k)
+        fn, dn = ind.first_collision(allb, a, len(inter), dirs)
+        code = np.where(fn < 0, -1, np.where(fn < len(inter), fn, -(fn - len(inter)) - 2))
+        assert np.array_equal(first, code), a
+        np.testing.assert_allclose(dtab, dn, rtol=1e-12)
+    w = np.geomspace(0.5, 60.0, 7)
+    coef, pows = [[1.0, 0.4], [0.7, 0.0]], [[5.0, 3.5], [4.0, 3.0]]
+    C = oracle.compute_c_integral(inter, ext, 8, 12, coef, pows, w, t_end=0.8, n_t=35, k_hat=k)
+    Cn = ind.c_integral(ri, re, 8, 12, coef, pows, w, 0.0, 0.8, 35)
+    assert rel_max(C, Cn) < 1e-12
+    # the spectrum of that frame: 8 pi w^3 (|C+|^2 + |Cx|^2)
+    S = oracle.spectrum(inter, ext, [k], 8, 12, coef, pows, w, t_end=0.8, n_t=35)
+    Sn = 8 * np.pi * w**3 * np.sum(np.abs(Cn) ** 2, axis=0)
+    np.testing.assert_allclose(S[0], Sn, rtol=1e-11)
+
+
+@pytest.mark.parametrize("seed", [3, 15])
+def test_integrand_matches_numpy_and_sums_to_integral(seed):
+    """compute_c_integrand (generalized_bulk_flow.rs:666-767, 875-920): each time step's
+    term against the restatement's per-step terms, and their sum equals
+    compute_c_integral with the unit cutoff."""
+    inter, ext = async_bubbles(seed)
+    w = np.geomspace(0.5, 60.0, 6)
+    coef, pows = [[1.0, 0.4], [0.7, 0.0]], [[5.0, 3.5], [4.0, 3.0]]
+    I = oracle.compute_c_integrand(inter, ext, 9, 12, coef, pows, w, t_begin=0.05, t_end=0.8,
+                                   n_t=33)
+    In = ind.c_integrand(inter, ext, 9, 12, coef, pows, w, 0.05, 0.8, 33)
+    assert I.shape == (2, 2, 6, 33)
+    scale = np.max(np.abs(In), axis=-1, keepdims=True)
+    assert np.max(np.abs(I - In) / scale) < 1e-12
+    C = oracle.compute_c_integral(inter, ext, 9, 12, coef, pows, w, t_begin=0.05, t_end=0.8,
+                                  n_t=33)
+    np.testing.assert_allclose(I.sum(axis=-1), C, rtol=0, atol=1e-13 * np.abs(C).max())
+    # a bubble contributes nothing at or before its nucleation time
+    late = int(np.argmax(inter[:, 0]))
+    t = np.linspace(0.0, 0.8, 33)
+    Il = oracle.compute_c_integrand(inter, ext, 9, 12, coef, pows, w, t_end=0.8, n_t=33,
+                                    selected_bubbles=[late])
+    assert np.all(Il[..., t <= inter[late, 0]] == 0)
+    assert np.any(Il[..., t > inter[late, 0]] != 0)
+
+
+def exponential_cutoff(t, smax, ratio_t_cut=None, ratio_t_0=None):
+    """ExponentialTimeCutoff as documented (time_cutoff.rs:14-43): weight 1 before
+    t_cut = r_cut smax (r_cut = 0.999999999 by default), a Gaussian of width
+    t_0 = r_0 (smax - t_cut) (r_0 = 0.25 by default) centred on t_cut after it."""
+    t_cut = (0.999999999 if ratio_t_cut is None else ratio_t_cut) * smax
+    t_0 = (0.25 if ratio_t_0 is None else ratio_t_0) * (smax - t_cut)
+    t = np.asarray(t, float)
+    return np.where(t < t_cut, 1.0, np.exp(-(((t - t_cut) / t_0) ** 2)))
+
+
+@pytest.mark.parametrize("cut", [(0.7, 0.6, 0.8), (0.8, None, None), (0.5, 0.3, None)])
+def test_exponential_cutoff_weights_each_step(cut):
+    """C with an exponential cutoff equals the sum over steps of the cutoff weight at
+    t_k times the step's integrand term, and the restatement with that weight."""
+    smax, rc, r0 = cut
+    inter, ext = async_bubbles(7)
+    w = np.geomspace(0.5, 60.0, 6)
+    coef, pows = [[1.0], [0.6]], [[5.0], [3.5]]
+    kw = dict(t_begin=0.0, t_end=0.8, n_t=41)
+    I = oracle.compute_c_integrand(inter, ext, 9, 12, coef, pows, w, **kw)
+    C = oracle.compute_c_integral(inter, ext, 9, 12, coef, pows, w, time_cutoff=(1, smax, rc, r0),
+                                  **kw)
+    t = np.linspace(0.0, 0.8, 41)
+    wt = exponential_cutoff(t, smax, rc, r0)
+    assert wt.min() < 0.99  # the cutoff acts inside the time range
+    np.testing.assert_allclose(C, np.sum(I * wt, axis=-1), rtol=0, atol=1e-13 * np.abs(C).max())
+    Cn = ind.c_integral(inter, ext, 9, 12, coef, pows, w, 0.0, 0.8, 41,
+                        cutoff=lambda tt: exponential_cutoff(tt, smax, rc, r0))
+    assert rel_max(C, Cn) < 1e-12
+    # the weight's shape: one at t_cut, 1/e one width later
+    t_cut = (0.999999999 if rc is None else rc) * smax
+    t_0 = (0.25 if r0 is None else r0) * (smax - t_cut)
+    np.testing.assert_allclose(exponential_cutoff([t_cut, t_cut + t_0], smax, rc, r0),
+                               [1.0, np.exp(-1.0)], rtol=1e-6)
+
+
 def test_errors_follow_reference():
     with pytest.raises(oracle.OracleError, match="formed inside bubble"):
         oracle.lattice(np.array([[0.0, 0, 0, 0], [1.0, 0.5, 0, 0]]))
@@ -253,7 +350,7 @@ def planted(tmp_path_factory):
     src = os.path.join(os.path.dirname(oracle.__file__), "bulkflow_oracle.cpp")
     out = {}
     d = tmp_path_factory.mktemp("planted")
-    for plant in (1, 2, 3):
+    for plant in (1, 2, 3, 4, 5):
         so = str(d / f"planted{plant}.so")
         subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
                         f"-DORACLE_PLANT={plant}", "-shared", "-o", so, src], check=True)
@@ -288,6 +385,20 @@ def test_planted_tie_mistake_is_caught(planted):
     inter = np.array([[0.0, 0.0, 0.0, 0.0], [0.0, 0.5, 0.5, 0.0], [0.0, 0.5, -0.5, 0.0]])
     first, _ = _with_lib(planted[3], lambda: oracle.first_colliding(inter, None, 3, 5, 0))
     assert first[1, 0] != -1
+
+
+@pytest.mark.parametrize("plant", [4, 5])
+def test_planted_frame_mistakes_are_caught(planted, plant):
+    """The k-hat frame rotated the wrong way, or positions rotated without the deltas:
+    the comparison with the restatement on pre-rotated bubbles fails."""
+    inter, ext = async_bubbles(4)
+    k = np.array([0.48, -0.6, 0.64])
+    w = np.geomspace(0.5, 60.0, 7)
+    R = oracle.rotation_to_z(k)
+    C = _with_lib(planted[plant], lambda: oracle.compute_c_integral(
+        inter, ext, 9, 12, [[1.0]], [[5.0]], w, t_end=0.8, n_t=35, k_hat=k))
+    Cn = ind.c_integral(rotate(inter, R), rotate(ext, R), 9, 12, [[1.0]], [[5.0]], w, 0.0, 0.8, 35)
+    assert rel_max(C, Cn) > 1e-6
 
 
 # ---------------------------------------------------------------------------
