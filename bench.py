@@ -132,14 +132,19 @@ def pmc_copy_calibration():
 
 
 def pmc_flops(mode: str, n_k: int):
-    """Executed FP64 VALU flops per phase-sum launch: SQ_INSTS_VALU_FLOPS_FP64
-    counts flops per wave instruction (an FMA counts 2), times 64 lanes."""
+    """Executed FP64 VALU flops per launch of the phase-sum and geometry kernels:
+    SQ_INSTS_VALU_FLOPS_FP64 counts flops per wave instruction (an FMA counts 2),
+    times 64 lanes.  Returns {"spectrum": F, "geometry": F} (None where missing)."""
     try:
         prog = ["--profile-mode", mode, "--steps", "2", "--warmup", "1", "--n-k", str(n_k)]
-        v = _mean_for(_pmc_run("SQ_INSTS_VALU_FLOPS_FP64", prog), "spectrum_kernel")
-        return None if v is None else 64.0 * v
+        res = _pmc_run("SQ_INSTS_VALU_FLOPS_FP64", prog)
+        out = {}
+        for key, needle in (("spectrum", "spectrum_kernel"), ("geometry", "geometry_kernel")):
+            v = _mean_for(res, needle)
+            out[key] = None if v is None else 64.0 * v
+        return out
     except Exception:
-        return None
+        return {"spectrum": None, "geometry": None}
 
 
 def pmc_traffic(mode: str, n_k: int, calibration):
@@ -430,7 +435,8 @@ def main():
         cal = pmc_copy_calibration()
         for m in ("fp64_blocked", "mixed"):
             traffic[m] = pmc_traffic(m, args.n_k, cal)
-        executed["fp64_blocked"] = pmc_flops("fp64_blocked", args.n_k)
+        for m in ("fp64_blocked", "fp64"):
+            executed[m] = pmc_flops(m, args.n_k)
 
     from gw_fopt_amd import many_bubbles as mb
     lb = mb.LatticeBubbles(mb.EmptyLattice(), wl.interior, wl.exterior)
@@ -459,7 +465,10 @@ def main():
     r = results[main_mode]
     value = points / (r["device_ms"] * 1e-3)
     spec_flops = flops["spectrum"]
-    achieved = spec_flops / (r["spectrum_ms"] * 1e-3) / 1e12
+    algorithmic_rate = spec_flops / (r["spectrum_ms"] * 1e-3) / 1e12
+    ex = executed.get(main_mode) or {}
+    ex_spec, ex_geo = ex.get("spectrum"), ex.get("geometry")
+    achieved = None if ex_spec is None else ex_spec / (r["spectrum_ms"] * 1e-3) / 1e12
     dtypes = {"fp64": "f64", "fp64_blocked": "f64",
               "mixed": "f32 sincos/accumulation, f64 masks and argument reduction"}
     tr = traffic.get(main_mode)
@@ -480,19 +489,27 @@ def main():
                                "images), 64x128 surface grid, 1000 t to percolation, 256 omegas "
                                "geomspace(0.5,500), 64 Fibonacci k-hat, f=ratio^5",
                    "form": main_mode, "n_bubbles": wl.n_bubbles, "n_k": len(wl.k_hat),
-                   "n_w": len(wl.w_arr), "n_t": wl.n_t, "grid": [wl.n_cos_thetax, wl.n_phix]},
+                   "n_w": len(wl.w_arr), "n_t": wl.n_t, "grid": [wl.n_cos_thetax, wl.n_phix],
+                   "timed_region": "device time of one whole spectrum call (geometry, phase "
+                                   "sums, reduction; HIP events on the library's stream) with "
+                                   "the bubbles resident in HBM; excludes the host-side call "
+                                   "set-up and the copies of omegas in and spectra out "
+                                   "(wall_points_per_s includes them)"},
         "roofline": {"bound": "valu", "kernel": f"spectrum_kernel ({main_mode} form)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "frac": None if achieved is None else achieved / FP64_PEAK_TFLOPS,
+                     "achieved_basis": "executed FP64 VALU flops per launch "
+                                       "(rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes, same "
+                                       "workload) / the kernel's event time",
                      "traffic": None if tr is None else tr["bytes"],
-                     "flops_per_launch": spec_flops, "launch_ms": r["spectrum_ms"],
+                     "executed_fp64_flops_per_launch": ex_spec, "launch_ms": r["spectrum_ms"],
                      "hbm_GBs": None if tr is None else tr["bytes"] / (r["spectrum_ms"] * 1e-3) / 1e9,
-                     "executed_fp64_flops_per_launch": executed.get(main_mode),
-                     "executed_tflops": (None if executed.get(main_mode) is None else
-                                         executed[main_mode] / (r["spectrum_ms"] * 1e-3) / 1e12),
-                     "executed_frac": (None if executed.get(main_mode) is None else
-                                       executed[main_mode] / (r["spectrum_ms"] * 1e-3) / 1e12
-                                       / FP64_PEAK_TFLOPS)},
+                     "algorithmic_flops_per_launch": spec_flops,
+                     "algorithmic_rate_tflops": algorithmic_rate,
+                     "algorithmic_note": "the workload's formula count 10 S_A + 10 S_C + 8 E_B "
+                                         "(one sincos per phase term) over the kernel time; the "
+                                         "factorised forms execute fewer operations, so this "
+                                         "rate is not a fraction of any peak"},
         "modes": {m: {"points_per_s": points / (v["device_ms"] * 1e-3),
                       "wall_points_per_s": points / (v["wall_ms"] * 1e-3),
                       "err_vs_fp64": errs.get(m), **v}
@@ -502,9 +519,15 @@ def main():
     if "fp64" in results:
         base = line["modes"]["fp64"]["points_per_s"]
         line["speedup_vs_fp64"] = {m: v["points_per_s"] / base for m, v in line["modes"].items()}
-    line["geometry"] = {"kernel": "geometry_kernel", "flops_per_launch": flops["geometry"],
-                        "launch_ms": r["geometry_ms"],
-                        "achieved_tflops": flops["geometry"] / (r["geometry_ms"] * 1e-3) / 1e12}
+    geo_ex = None if ex_geo is None else ex_geo / (r["geometry_ms"] * 1e-3) / 1e12
+    line["geometry"] = {"kernel": "geometry_kernel", "launch_ms": r["geometry_ms"],
+                        "executed_fp64_flops_per_launch": ex_geo,
+                        "executed_tflops": geo_ex,
+                        "frac": None if geo_ex is None else geo_ex / FP64_PEAK_TFLOPS,
+                        "algorithmic_flops_per_launch": flops["geometry"],
+                        "algorithmic_rate_tflops": flops["geometry"] / (r["geometry_ms"] * 1e-3) / 1e12,
+                        "algorithmic_note": "10 flops per (bubble, direction, candidate) test; "
+                                            "the key-ordered walks test far fewer candidates"}
     if traffic:
         line["pmc_traffic"] = traffic
     if not args.no_grad:

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libbubble_gw_c.so")
 # BgwStatus (include/bubble_gw.h)
 OK, NULL_POINTER, INVALID_ARGUMENT, CALCULATION_ERROR, PANIC = range(5)
 # BgwPrecision
-FP64, MIXED, FP64_BLOCKED = 0, 1, 2
+FP64, MIXED, FP64_BLOCKED, FP64_MATRIX = 0, 1, 2, 3
 # BgwTimeCutoffKind
 CUTOFF_UNIT, CUTOFF_EXPONENTIAL = 0, 1
 

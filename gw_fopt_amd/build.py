@@ -22,7 +22,11 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libbubble_gw_c.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["bgw_kernels.hip", "bgw_ensemble.hip", "bgw_nucleation.hip", "bgw_host.cpp"]
+SOURCES = ["bgw_kernels.hip", "bgw_ensemble.hip", "bgw_nucleation.hip", "bgw_matrix.hip",
+           "bgw_host.cpp"]
+# per-source flags: the matrix-core kernels keep their accumulators in VGPRs
+# (the AGPR form copies every accumulator out and back around each step)
+EXTRA_FLAGS = {"bgw_matrix.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 OFFLOAD_ARCH = "gfx950"
 
 
@@ -53,7 +57,8 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
         obj = os.path.join(objdir, src + ".o")
         objs.append(obj)
         lang = ["-x", "hip", f"--offload-arch={OFFLOAD_ARCH}"]
-        cmd = [cc, *lang, *common_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *lang, *common_flags(), *EXTRA_FLAGS.get(src, []), "-c",
+               os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -75,7 +75,8 @@ std::array<double, 9> rotation_to_z(const double* k) {
 }
 
 bool valid_precision(BgwPrecision p) {
-  return p == BgwPrecision_Fp64 || p == BgwPrecision_Mixed || p == BgwPrecision_Fp64Blocked;
+  return p == BgwPrecision_Fp64 || p == BgwPrecision_Mixed || p == BgwPrecision_Fp64Blocked ||
+         p == BgwPrecision_Fp64Matrix;
 }
 
 // Phase-sum kernel variant of a precision.
@@ -84,6 +85,7 @@ int kernel_mode(BgwPrecision p) {
     case BgwPrecision_Fp64: return 0;
     case BgwPrecision_Fp64Blocked: return 1;
     case BgwPrecision_Mixed: return 2;
+    case BgwPrecision_Fp64Matrix: return 3;
   }
   throw CalcError("precision is not a BgwPrecision");
 }
@@ -370,7 +372,7 @@ struct Engine {
   // per call
   DevBuf d_items, d_order, d_rot, d_sin_t, d_cos_t, d_cos_p, d_sin_p, d_gcos, d_gsin, d_ring_c,
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
-      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
+      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc, d_tiles;
   // batched ensemble: work units, queue counter, statistics, captured graph
   DevBuf d_units, d_unit_off, d_counter, d_kmean, d_emean, d_estd;
   hipGraphExec_t graph = nullptr;
@@ -635,6 +637,57 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   ra.s_out = E.d_s.as<double>();
 }
 
+// The FP64 matrix form for the sets of sb: tiles of the items of one
+// (realisation, reference bubble) in consecutive k-hat slots, most expensive
+// first (earliest nucleation).  Launch shape: two column tiles when a bubble has
+// more items than one column tile holds; 16-omega tiles per matrix wavefront
+// from q.tile_rb (1 or 2; 0: 1).
+void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw::SpecArgs& sb,
+                        hipStream_t st) {
+  if (sb.has_damping)
+    throw CalcError("the FP64 matrix form does not take a damping width (use fp64_blocked)");
+  if (q.n_t > 65535) throw CalcError("the FP64 matrix form supports n_t <= 65535");
+  if (q.tile_rb != 0 && q.tile_rb != 1 && q.tile_rb != 2)
+    throw CalcError("tiles.block_steps of the FP64 matrix form: 0, 1 or 2 omega tiles per wavefront");
+  const size_t n_k = q.k.size();
+  bgw::MatArgs ma{};
+  ma.s = sb;
+  ma.col_tiles = static_cast<size_t>(bgw::matrix_items_per_tile(sb.n_sets, 1)) < n_k ? 2 : 1;
+  ma.omega_tiles = q.tile_rb ? q.tile_rb : 1;
+  ma.ni = std::min<int>(bgw::matrix_items_per_tile(sb.n_sets, ma.col_tiles), static_cast<int>(n_k));
+  // wide rings: fewer items per tile, so that the tables fit the 160 KiB of LDS
+  while (ma.ni > 1 && bgw::matrix_lds_bytes(sb, ma.ni, ma.col_tiles) > 160 * 1024) --ma.ni;
+  if (bgw::matrix_lds_bytes(sb, ma.ni, ma.col_tiles) > 160 * 1024)
+    throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+  std::vector<int32_t> tiles;
+  std::vector<std::pair<double, int32_t>> cost;
+  const size_t n_real = E.real.size();
+  for (size_t r = 0; r < n_real; ++r) {
+    const size_t n_sel = q.bubbles_of_real[r].size();
+    for (size_t p = 0; p < n_sel; ++p)
+      for (size_t k0 = 0; k0 < n_k; k0 += static_cast<size_t>(ma.ni)) {
+        const int32_t first = Lc.group_off[r * n_k + k0] + static_cast<int32_t>(p);
+        cost.emplace_back(Lc.item_cost[first], static_cast<int32_t>(cost.size()));
+        for (int j = 0; j < ma.ni; ++j) {
+          const size_t k = k0 + static_cast<size_t>(j);
+          tiles.push_back(k < n_k ? Lc.group_off[r * n_k + k] + static_cast<int32_t>(p) : -1);
+        }
+      }
+  }
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<double, int32_t>& x, const std::pair<double, int32_t>& y) {
+                     return x.first > y.first;
+                   });
+  std::vector<int32_t> ordered;
+  ordered.reserve(tiles.size());
+  for (const auto& c : cost)
+    ordered.insert(ordered.end(), tiles.begin() + static_cast<size_t>(c.second) * ma.ni,
+                   tiles.begin() + static_cast<size_t>(c.second + 1) * ma.ni);
+  upload(E.d_tiles, ordered, st);
+  ma.tiles = E.d_tiles.as<int32_t>();
+  HIPCHECK(bgw::launch_matrix(ma, static_cast<int>(cost.size()), st));
+}
+
 Result run(Engine& E, const Request& q) {
   const Resolution& res = *q.res;
   const Scaling& sc = *q.sc;
@@ -666,6 +719,12 @@ Result run(Engine& E, const Request& q) {
     sb.coef = E.d_coef.as<double>() + s0 * sc.n_terms;
     sb.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
     sb.time_tables = 0;
+    if (q.mode == 3) {
+      launch_matrix_form(E, q, Lc, sb, st);
+      continue;
+    }
+    if (q.tile_rb != 0 && q.tile_rb != 8 && q.tile_rb != 16)
+      throw CalcError("tiles.block_steps must be 0, 8 or 16 for this form");
     sb.time_tables = bgw::spectrum_time_tables(sb) ? 1 : 0;
     if (bgw::spectrum_lds_bytes(sb, q.mode) > 160 * 1024)
       throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
@@ -1253,8 +1312,9 @@ struct bgw_ensemble {
 namespace {
 
 void check_tiles(const BgwTileConfig& t) {
-  if (t.block_steps != 0 && t.block_steps != 8 && t.block_steps != 16)
-    throw CalcError("tiles.block_steps must be 0, 8 or 16");
+  if (t.block_steps != 0 && t.block_steps != 1 && t.block_steps != 2 && t.block_steps != 8 &&
+      t.block_steps != 16)
+    throw CalcError("tiles.block_steps must be 0, 8 or 16 (1 or 2 for the FP64 matrix form)");
   if (t.threads != 0 && t.threads != 64 && t.threads != 128 && t.threads != 256)
     throw CalcError("tiles.threads must be 0, 64, 128 or 256");
   if (t.persistent_blocks > 65536) throw CalcError("tiles.persistent_blocks must be <= 65536");
@@ -1481,6 +1541,9 @@ Request ensemble_request(bgw_ensemble* ensemble, const double* w_arr, size_t n_w
   slice_from_raw(t_begin, n_real, "t_begin");
   slice_from_raw(t_end, n_real, "t_end");
   if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
+  if (precision == BgwPrecision_Fp64Matrix)
+    throw CalcError("the FP64 matrix form runs single configurations only (not the batched "
+                    "ensembles): use fp64_blocked or mixed");
   check_unit_vectors(k_hat, n_k);
   Request q;
   q.res = &need_resolution(ensemble->res);

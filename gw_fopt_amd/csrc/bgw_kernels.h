@@ -110,6 +110,25 @@ size_t ensemble_lds_bytes(const EnsArgs& a, int mode);
 hipError_t launch_ensemble_stats(const double* s_spec, int n_real, int n_k, int nq, double* kmean,
                                  double* mean, double* stdev, hipStream_t s);
 
+// The FP64 matrix form of the phase sums (bgw_matrix.hip): workgroups of
+// kMxMfmaWaves matrix wavefronts and kMxProdWaves producer wavefronts, one per
+// tile of `ni` items of one (realisation, reference bubble) and block of
+// kMxMfmaWaves * 16 * omega_tiles omegas.  tiles: (n_tiles, ni) item indices,
+// -1 padding; every item of a tile has the same t_a.
+constexpr int kMxMfmaWaves = 8;
+constexpr int kMxProdWaves = 4;
+constexpr int kMatrixThreads = 64 * (kMxMfmaWaves + kMxProdWaves);
+struct MatArgs {
+  SpecArgs s;             // the phase-sum arguments (no damping; n_t <= 65535)
+  const int32_t* tiles;
+  int ni;                 // items per tile, <= matrix_items_per_tile(s.n_sets, col_tiles)
+  int omega_tiles;        // 16-omega tiles per matrix wavefront: 1 or 2
+  int col_tiles;          // 16-column tiles per workgroup: 1 or 2
+};
+int matrix_items_per_tile(int n_sets, int col_tiles);
+size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles);
+hipError_t launch_matrix(const MatArgs& a, int n_tiles, hipStream_t s);
+
 // Default launch shape of the phase sums (0 in SpecArgs::mixed_block /
 // threads): time steps per factorised block, 8 for the FP64 blocked form and
 // 16 otherwise; workgroups of 64..256 lanes holding one omega each in the

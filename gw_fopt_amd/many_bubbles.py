@@ -42,7 +42,9 @@ __all__ = [
     "NewGeneralizedBulkFlow", "fibonacci_directions",
 ]
 
-_PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED, "fp64_blocked": _lib.FP64_BLOCKED}
+_PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED, "fp64_blocked": _lib.FP64_BLOCKED,
+               "fp64_matrix": _lib.FP64_MATRIX}
+_PRECISION_NAMES = "'fp64', 'fp64_blocked', 'fp64_matrix' or 'mixed'"
 
 
 def _precision(p) -> int:
@@ -50,10 +52,10 @@ def _precision(p) -> int:
         try:
             return _PRECISIONS[p.lower()]
         except KeyError:
-            raise ValueError(f"precision must be 'fp64', 'fp64_blocked' or 'mixed', got {p!r}") from None
-    if p in (_lib.FP64, _lib.MIXED, _lib.FP64_BLOCKED):
+            raise ValueError(f"precision must be {_PRECISION_NAMES}, got {p!r}") from None
+    if p in tuple(_PRECISIONS.values()):
         return int(p)
-    raise ValueError(f"precision must be 'fp64', 'fp64_blocked' or 'mixed', got {p!r}")
+    raise ValueError(f"precision must be {_PRECISION_NAMES}, got {p!r}")
 
 
 # ---------------------------------------------------------------------------

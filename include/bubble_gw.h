@@ -69,11 +69,18 @@ typedef struct bgw_generalized_bulk_flow bgw_generalized_bulk_flow;
  *               phases of R consecutive time steps factorised as anchor x
  *               table, sincos, tables, weights and block sums in float, block
  *               results accumulated in float with Kahan compensation.
- *  Fp64Blocked: the same block factorisation entirely in double. */
+ *  Fp64Blocked: the same block factorisation entirely in double.
+ *  Fp64Matrix:  double throughout, on the matrix cores: the phase sums of the
+ *               items of one bubble in several k-hat frames (same t_a) are one
+ *               matrix product per ring (phase factors x weights), accumulated
+ *               by FP64 MFMA; fast when many k-hat share the call (spectra);
+ *               no damping width; n_t <= 65535; single configurations only
+ *               (not the batched ensembles). */
 typedef enum BgwPrecision {
     BgwPrecision_Fp64 = 0,
     BgwPrecision_Mixed = 1,
     BgwPrecision_Fp64Blocked = 2,
+    BgwPrecision_Fp64Matrix = 3,
 } BgwPrecision;
 
 /* Device times of the most recent compute call on a handle (HIP events on

@@ -183,3 +183,11 @@ def test_setters_and_new_directions_between_calls(ens, chunk):
         got = E.compute_spectrum(w, kk, t_end=t_end, n_t=N_T, precision="fp64", tiles=tl)["S"]
         want = fresh(n_c, n_phi, coef, pows, damping, kk)
         assert got.tobytes() == want.tobytes(), (n_c, n_phi, coef, damping)
+
+
+def test_matrix_form_is_refused_for_ensembles(ens):
+    inter, ext, t_end, w, k = ens
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    with pytest.raises(ValueError, match="single configurations only"):
+        E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")

@@ -77,9 +77,13 @@ def test_reference_bench_configuration():
             assert rel_to_max(C[pm, s], Co[pm, s]) <= FP64_REL, (pm, s)
     k = mb.fibonacci_directions(2)
     So = oracle.spectrum(inter, None, k, 50, 100, coef, pows, w, t_begin=0.0, t_end=15.0, n_t=800)
-    for precision, bar in (("fp64", FP64_REL), ("fp64_blocked", FP64_REL)):
+    for precision in ("fp64", "fp64_blocked", "fp64_matrix"):
         S, _ = bf.compute_spectrum(w, k, t_begin=0.0, t_end=15.0, n_t=800, precision=precision)
-        assert rel_per_bin(S, So) <= bar, precision
+        assert rel_per_bin(S, So) <= FP64_REL, precision
+    Cm = bf.compute_c_integral(w, t_begin=0.0, t_end=15.0, n_t=800, precision="fp64_matrix")
+    for pm in range(2):
+        for s in range(2):
+            assert rel_to_max(Cm[pm, s], Co[pm, s]) <= FP64_REL, (pm, s)
 
 
 def test_masks_bitexact_plain_8():
@@ -155,13 +159,22 @@ def test_c_integral_async_multiset_cutoff(seed):
     w = np.geomspace(0.5, 120.0, 33)
     tc = mb.ExponentialTimeCutoff(0.7, 0.8, None)
     bf = flow(inter, ext, 16, 24, coef, pows)
-    C = bf.compute_c_integral(w, t_begin=0.0, t_end=0.7, n_t=90, time_cutoff=tc)
     Co = oracle.compute_c_integral(inter, ext, 16, 24, coef, pows, w, t_begin=0.0, t_end=0.7,
                                    n_t=90, time_cutoff=(1, 0.7, 0.8, None))
-    S = np.sum(np.abs(C) ** 2, axis=0)
     So = np.sum(np.abs(Co) ** 2, axis=0)
-    assert rel_per_bin(S[1:], So[1:]) <= FP64_REL
-    assert rel_to_max(C, Co) <= FP64_REL
+    for precision in ("fp64", "fp64_matrix"):
+        C = bf.compute_c_integral(w, t_begin=0.0, t_end=0.7, n_t=90, time_cutoff=tc,
+                                  precision=precision)
+        S = np.sum(np.abs(C) ** 2, axis=0)
+        assert rel_per_bin(S[1:], So[1:]) <= FP64_REL, precision
+        assert rel_to_max(C, Co) <= FP64_REL, precision
+    # three sets in a k-hat spectrum: tiles of fewer items, sets in one launch
+    k = mb.fibonacci_directions(5)
+    Sk = oracle.spectrum(inter, ext, k, 16, 24, coef, pows, w, t_begin=0.0, t_end=0.7, n_t=90,
+                         time_cutoff=(1, 0.7, 0.8, None))
+    Sm, _ = bf.compute_spectrum(w, k, t_begin=0.0, t_end=0.7, n_t=90, time_cutoff=tc,
+                                precision="fp64_matrix")
+    assert rel_per_bin(Sm[:, 1:], Sk[:, 1:]) <= FP64_REL
 
 
 def test_c_integral_damping():
@@ -172,15 +185,20 @@ def test_c_integral_damping():
     Co = oracle.compute_c_integral(inter, ext, 10, 16, [[1.0]], [[4.0]], w, t_end=0.6, n_t=60,
                                    damping=0.3)
     assert rel_per_bin(np.sum(np.abs(C) ** 2, axis=0), np.sum(np.abs(Co) ** 2, axis=0)) <= FP64_REL
+    # the matrix form takes no damping width: a clear error, nothing computed
+    with pytest.raises(ValueError, match="does not take a damping width"):
+        bf.compute_c_integral(w, t_end=0.6, n_t=60, precision="fp64_matrix")
 
 
-def test_spectrum_plain_8_fp64_blocked():
-    """Block-factorised FP64 phases meet the plain form's parity bar."""
+@pytest.mark.parametrize("precision", ["fp64_blocked", "fp64_matrix"])
+def test_spectrum_plain_8_fp64_factorised(precision):
+    """Block-factorised FP64 phases, and the matrix-core form, meet the plain form's
+    parity bar."""
     wl = configs.plain_8()
     bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
               wl.powers_sets)
     S, _ = bf.compute_spectrum(wl.w_arr, wl.k_hat, t_end=wl.t_end, n_t=wl.n_t,
-                               precision="fp64_blocked")
+                               precision=precision)
     So = oracle.spectrum(wl.interior, wl.exterior, wl.k_hat, wl.n_cos_thetax, wl.n_phix,
                          wl.coefficients_sets, wl.powers_sets, wl.w_arr, t_end=wl.t_end,
                          n_t=wl.n_t)
@@ -221,6 +239,8 @@ def test_spectrum_multi_chunk_blocked_and_mixed():
     assert rel_per_bin(S64, So) <= FP64_REL
     Sb, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="fp64_blocked")
     assert rel_per_bin(Sb, So) <= FP64_REL
+    Sm, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="fp64_matrix")
+    assert rel_per_bin(Sm, So) <= FP64_REL
     S32, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=1100, precision="mixed")
     rel = np.abs(S32 - S64) / np.abs(S64)
     # 5.5x more time steps than plain_8: the cancellation floor of the float
@@ -308,9 +328,10 @@ def test_masks_bitexact_every_sort_width(n_phi):
         np.testing.assert_array_equal(f, fo)
         assert d.tobytes() == do.tobytes()
     w = np.geomspace(0.5, 40.0, 6)
-    S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision="fp64")
     So = oracle.spectrum(inter, ext, [k], 5, n_phi, [[1.0]], [[5.0]], w, t_end=0.9, n_t=40)
-    assert rel_per_bin(S, So) <= FP64_REL
+    for precision in ("fp64", "fp64_matrix"):
+        S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision=precision)
+        assert rel_per_bin(S, So) <= FP64_REL, precision
 
 
 @pytest.mark.parametrize("n_c", [9, 17, 33, 64])
@@ -359,6 +380,8 @@ def test_masks_and_spectra_random_configurations():
         np.testing.assert_allclose(S, So, rtol=FP64_REL, atol=0)
         Sb, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="fp64_blocked")
         np.testing.assert_allclose(Sb, So, rtol=FP64_REL, atol=0)
+        Smx, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="fp64_matrix")
+        np.testing.assert_allclose(Smx, So, rtol=FP64_REL, atol=0)
         Sm, _ = bf.compute_spectrum(w, [k], t_end=t_end, n_t=60, precision="mixed")
         np.testing.assert_allclose(Sm, S, rtol=1e-4, atol=0)
 
@@ -375,12 +398,14 @@ def test_masks_bitexact_wide_rings(n_phi):
         np.testing.assert_array_equal(f, fo)
         assert d.tobytes() == do.tobytes()
     w = np.geomspace(0.5, 40.0, 6)
-    S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision="fp64")
     So = oracle.spectrum(inter, ext, [k], 4, n_phi, [[1.0]], [[5.0]], w, t_end=0.9, n_t=40)
-    assert rel_per_bin(S, So) <= FP64_REL
+    for precision in ("fp64", "fp64_matrix"):
+        S, _ = bf.compute_spectrum(w, [k], t_end=0.9, n_t=40, precision=precision)
+        assert rel_per_bin(S, So) <= FP64_REL, precision
 
 
-@pytest.mark.parametrize("precision,bar", [("fp64", FP64_REL), ("fp64_blocked", FP64_REL)])
+@pytest.mark.parametrize("precision,bar", [("fp64", FP64_REL), ("fp64_blocked", FP64_REL),
+                                           ("fp64_matrix", FP64_REL)])
 def test_spectrum_many_omegas(precision, bar):
     """600 omegas: three omegas per lane of a 256-lane workgroup."""
     wl = configs.plain_8()

@@ -4,8 +4,8 @@
   bubble, 1000 steps to percolation, 256 omegas in [0.5, 500]) in two k-hat
   frames: every bubble's first colliders and delays bit-exact (digests of the
   oracle's, and the oracle run live on a subset for a readable diff), S within
-  1e-10 per bin and C within 1e-10 of each row's largest entry, for the FP64
-  and the blocked FP64 forms.  This is where the candidate walks of the
+  1e-10 per bin and C within 1e-10 of each row's largest entry, for the FP64,
+  the blocked FP64 and the FP64 matrix forms.  This is where the candidate walks of the
   geometry kernel are longest and where the uncollided-ring shortcut meets
   the highest omegas.
 * Four realisations of the 512-member ensemble at its grids (32 x 64, 500
@@ -78,7 +78,7 @@ def test_plain_256_masks_bitexact(p256):
     assert not bad, f"first colliders / delays differ from the oracle's for {bad[:10]}"
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp64_blocked"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_blocked", "fp64_matrix"])
 def test_plain_256_spectrum(p256, precision):
     g, bf = p256
     S, M, C = bf.compute_spectrum(g["w"], g["k_hat"], t_end=float(g["t_end"]), n_t=int(g["n_t"]),
@@ -139,3 +139,18 @@ def test_ensemble_members_full_size(members, precision, tiles):
         S, _ = bf.compute_spectrum(g["w"][r], g["k_hat"], t_end=float(g["t_end"][r]), n_t=n_t,
                                    precision=precision)
         assert out["S"][r].tobytes() == S.tobytes(), r
+
+
+def test_ensemble_members_matrix_form(members):
+    """The FP64 matrix form on each of the four realisations alone, against the oracle."""
+    g = members
+    for r in range(len(g["members"])):
+        bf = mb.GeneralizedBulkFlow(mb.LatticeBubbles(mb.EmptyLattice(), g[f"interior_{r}"],
+                                                      g[f"exterior_{r}"]))
+        bf.set_resolution(int(g["n_c"]), int(g["n_phi"]))
+        bf.set_gradient_scaling_params(g["coef"].tolist(), g["pows"].tolist())
+        S, _ = bf.compute_spectrum(g["w"][r], g["k_hat"], t_end=float(g["t_end"][r]),
+                                   n_t=int(g["n_t"]), precision="fp64_matrix")
+        err = rel_per_bin(S, g[f"S_{r}"])
+        print(f"member {int(g['members'][r])}: fp64_matrix max per-bin rel err {err:.2e}")
+        assert err <= FP64_REL, r

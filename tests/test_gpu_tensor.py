@@ -32,6 +32,8 @@ def test_tensor_plain_8_matches_oracle():
     assert _rel_per_component(T, To) <= 1e-10
     Tb = bf.compute_c_tensor(w, t_end=wl.t_end, n_t=wl.n_t, precision="fp64_blocked")
     assert _rel_per_component(Tb, To) <= 1e-10
+    Tm = bf.compute_c_tensor(w, t_end=wl.t_end, n_t=wl.n_t, precision="fp64_matrix")
+    assert _rel_per_component(Tm, To) <= 1e-10
     # the plain form's projections from the same handle
     C = bf.compute_c_integral(w, t_end=wl.t_end, n_t=wl.n_t)
     assert np.max(np.abs((T[0] - T[3]) / 2 - C[0])) <= 1e-10 * np.max(np.abs(C[0]))
