@@ -372,7 +372,9 @@ struct Engine {
   // per call
   DevBuf d_items, d_order, d_rot, d_sin_t, d_cos_t, d_cos_p, d_sin_p, d_gcos, d_gsin, d_ring_c,
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
-      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc, d_tiles;
+      d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
+  // FP64 matrix form: tiles, their time tables and weight rows
+  DevBuf d_tiles, d_tinfo, d_tt, d_kstart, d_ctlo, d_vbuf;
   // batched ensemble: work units, queue counter, statistics, captured graph
   DevBuf d_units, d_unit_off, d_counter, d_kmean, d_emean, d_estd;
   hipGraphExec_t graph = nullptr;
@@ -473,6 +475,7 @@ struct Launch {
   std::vector<Item> items;               // in (realisation, k-hat, bubble) order
   std::vector<double> item_cost;         // relative phase-sum cost of each item
   std::vector<int32_t> group_off, group_r;
+  std::vector<RealInfo> real;            // time grids and omega offsets of this request
 };
 
 void setup(Engine& E, const Request& q, Launch& Lc) {
@@ -540,6 +543,7 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   Lc.item_cost = cost;
   Lc.group_off = group_off;
   Lc.group_r = group_r;
+  Lc.real = real;
   upload(E.d_real, real, st);
   upload(E.d_items, items, st);
   upload(E.d_order, order, st);
@@ -639,31 +643,37 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
 
 // The FP64 matrix form for the sets of sb: tiles of the items of one
 // (realisation, reference bubble) in consecutive k-hat slots, most expensive
-// first (earliest nucleation).  Launch shape: two column tiles when a bubble has
-// more items than one column tile holds; 16-omega tiles per matrix wavefront
-// from q.tile_rb (1 or 2; 0: 1).
+// first (earliest nucleation); their weight rows go through HBM in batches of
+// at most kMatrixRowBytes.  Launch shape: two column tiles when a bubble has
+// more items than one column tile holds; 16-omega tiles per wavefront of the
+// matrix kernel from q.tile_rb (1, 2 or 4; 0: 2).
+constexpr size_t kMatrixRowBytes = size_t(16) << 30;
+
 void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw::SpecArgs& sb,
                         hipStream_t st) {
   if (sb.has_damping)
     throw CalcError("the FP64 matrix form does not take a damping width (use fp64_blocked)");
   if (q.n_t > 65535) throw CalcError("the FP64 matrix form supports n_t <= 65535");
-  if (q.tile_rb != 0 && q.tile_rb != 1 && q.tile_rb != 2)
-    throw CalcError("tiles.block_steps of the FP64 matrix form: 0, 1 or 2 omega tiles per wavefront");
+  if (q.tile_rb != 0 && q.tile_rb != 1 && q.tile_rb != 2 && q.tile_rb != 4)
+    throw CalcError("tiles.block_steps of the FP64 matrix form: 0, 1, 2 or 4 omega tiles per wavefront");
   const size_t n_k = q.k.size();
   bgw::MatArgs ma{};
   ma.s = sb;
   ma.col_tiles = static_cast<size_t>(bgw::matrix_items_per_tile(sb.n_sets, 1)) < n_k ? 2 : 1;
-  ma.omega_tiles = q.tile_rb ? q.tile_rb : 1;
+  ma.omega_tiles = q.tile_rb ? q.tile_rb : 2;
   ma.ni = std::min<int>(bgw::matrix_items_per_tile(sb.n_sets, ma.col_tiles), static_cast<int>(n_k));
   // wide rings: fewer items per tile, so that the tables fit the 160 KiB of LDS
   while (ma.ni > 1 && bgw::matrix_lds_bytes(sb, ma.ni, ma.col_tiles) > 160 * 1024) --ma.ni;
   if (bgw::matrix_lds_bytes(sb, ma.ni, ma.col_tiles) > 160 * 1024)
     throw CalcError("resolution too large: the per-ring tables exceed the 160 KiB of LDS");
+  const size_t ncol = 16 * static_cast<size_t>(ma.col_tiles);
   std::vector<int32_t> tiles;
   std::vector<std::pair<double, int32_t>> cost;
+  std::vector<int32_t> kbase_of;
   const size_t n_real = E.real.size();
   for (size_t r = 0; r < n_real; ++r) {
     const size_t n_sel = q.bubbles_of_real[r].size();
+    const RealInfo& ri = Lc.real[r];
     for (size_t p = 0; p < n_sel; ++p)
       for (size_t k0 = 0; k0 < n_k; k0 += static_cast<size_t>(ma.ni)) {
         const int32_t first = Lc.group_off[r * n_k + k0] + static_cast<int32_t>(p);
@@ -672,20 +682,68 @@ void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw
           const size_t k = k0 + static_cast<size_t>(j);
           tiles.push_back(k < n_k ? Lc.group_off[r * n_k + k] + static_cast<int32_t>(p) : -1);
         }
+        // first step with t_k - t_a > 0, as the kernels compute t_k
+        const double t_a = Lc.items[first].t_a;
+        int lo = 0, hi = static_cast<int>(q.n_t);
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if ((ri.t_begin + ri.t_step * static_cast<double>(mid)) - t_a > 0.0) hi = mid;
+          else lo = mid + 1;
+        }
+        kbase_of.push_back(lo & ~3);
       }
   }
   std::stable_sort(cost.begin(), cost.end(),
                    [](const std::pair<double, int32_t>& x, const std::pair<double, int32_t>& y) {
                      return x.first > y.first;
                    });
+  const size_t n_tiles = cost.size();
   std::vector<int32_t> ordered;
+  std::vector<bgw::MxTileInfo> info(n_tiles);
   ordered.reserve(tiles.size());
-  for (const auto& c : cost)
-    ordered.insert(ordered.end(), tiles.begin() + static_cast<size_t>(c.second) * ma.ni,
-                   tiles.begin() + static_cast<size_t>(c.second + 1) * ma.ni);
+  int64_t voff = 0;
+  for (size_t t = 0; t < n_tiles; ++t) {
+    const int32_t c = cost[t].second;
+    ordered.insert(ordered.end(), tiles.begin() + static_cast<size_t>(c) * ma.ni,
+                   tiles.begin() + static_cast<size_t>(c + 1) * ma.ni);
+    info[t].kbase = kbase_of[c];
+    info[t].len = (static_cast<int32_t>(q.n_t) - kbase_of[c] + 3) & ~3;
+    info[t].voff = voff;
+    voff += static_cast<int64_t>(sb.n_c) * info[t].len * static_cast<int64_t>(ncol);
+  }
   upload(E.d_tiles, ordered, st);
+  upload(E.d_tinfo, info, st);
+  const size_t nterms = static_cast<size_t>(sb.n_sets) * sb.n_terms;
+  E.d_tt.ensure(std::max<size_t>(1, n_tiles * q.n_t * (1 + nterms)) * sizeof(double));
+  E.d_kstart.ensure(std::max<size_t>(1, n_tiles * sb.n_c) * sizeof(int32_t));
+  E.d_ctlo.ensure(std::max<size_t>(1, n_tiles * sb.n_c * 2) * sizeof(int32_t));
   ma.tiles = E.d_tiles.as<int32_t>();
-  HIPCHECK(bgw::launch_matrix(ma, static_cast<int>(cost.size()), st));
+  ma.tinfo = E.d_tinfo.as<bgw::MxTileInfo>();
+  ma.tt = E.d_tt.as<double>();
+  ma.kstart = E.d_kstart.as<int32_t>();
+  ma.ctlo = E.d_ctlo.as<int32_t>();
+  // batches of whole tiles within the row budget (a tile larger than the budget alone)
+  std::vector<std::pair<size_t, size_t>> batches;
+  size_t max_batch = 1;
+  for (size_t t0 = 0; t0 < n_tiles;) {
+    size_t t1 = t0 + 1;
+    while (t1 < n_tiles &&
+           static_cast<size_t>(info[t1].voff + static_cast<int64_t>(sb.n_c) * info[t1].len *
+                               static_cast<int64_t>(ncol) - info[t0].voff) * sizeof(double) <=
+               kMatrixRowBytes)
+      ++t1;
+    const int64_t end = t1 < n_tiles ? info[t1].voff : voff;
+    max_batch = std::max<size_t>(max_batch, static_cast<size_t>(end - info[t0].voff));
+    batches.emplace_back(t0, t1);
+    t0 = t1;
+  }
+  E.d_vbuf.ensure(max_batch * sizeof(double));
+  ma.vbuf = E.d_vbuf.as<double>();
+  for (const auto& b : batches) {
+    ma.tile0 = static_cast<int>(b.first);
+    ma.voff0 = info[b.first].voff;
+    HIPCHECK(bgw::launch_matrix(ma, static_cast<int>(b.second - b.first), st));
+  }
 }
 
 Result run(Engine& E, const Request& q) {
@@ -720,7 +778,19 @@ Result run(Engine& E, const Request& q) {
     sb.pows = E.d_pows.as<double>() + s0 * sc.n_terms;
     sb.time_tables = 0;
     if (q.mode == 3) {
-      launch_matrix_form(E, q, Lc, sb, st);
+      // sets of one matrix-form launch: at most kMaxMatrixTerms (set, term) pairs
+      if (sc.n_terms > static_cast<size_t>(bgw::kMaxMatrixTerms))
+        throw CalcError("the FP64 matrix form takes at most " +
+                        std::to_string(bgw::kMaxMatrixTerms) + " terms per set");
+      const size_t per = std::max<size_t>(1, bgw::kMaxMatrixTerms / sc.n_terms);
+      for (size_t s1 = 0; s1 < static_cast<size_t>(sb.n_sets); s1 += per) {
+        bgw::SpecArgs sm = sb;
+        sm.set0 = sb.set0 + static_cast<int>(s1);
+        sm.n_sets = static_cast<int>(std::min(per, static_cast<size_t>(sb.n_sets) - s1));
+        sm.coef = sb.coef + s1 * sc.n_terms;
+        sm.pows = sb.pows + s1 * sc.n_terms;
+        launch_matrix_form(E, q, Lc, sm, st);
+      }
       continue;
     }
     if (q.tile_rb != 0 && q.tile_rb != 8 && q.tile_rb != 16)
@@ -1312,9 +1382,9 @@ struct bgw_ensemble {
 namespace {
 
 void check_tiles(const BgwTileConfig& t) {
-  if (t.block_steps != 0 && t.block_steps != 1 && t.block_steps != 2 && t.block_steps != 8 &&
-      t.block_steps != 16)
-    throw CalcError("tiles.block_steps must be 0, 8 or 16 (1 or 2 for the FP64 matrix form)");
+  if (t.block_steps != 0 && t.block_steps != 1 && t.block_steps != 2 && t.block_steps != 4 &&
+      t.block_steps != 8 && t.block_steps != 16)
+    throw CalcError("tiles.block_steps must be 0, 8 or 16 (1, 2 or 4 for the FP64 matrix form)");
   if (t.threads != 0 && t.threads != 64 && t.threads != 128 && t.threads != 256)
     throw CalcError("tiles.threads must be 0, 64, 128 or 256");
   if (t.persistent_blocks > 65536) throw CalcError("tiles.persistent_blocks must be <= 65536");

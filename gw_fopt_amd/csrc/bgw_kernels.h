@@ -110,21 +110,33 @@ size_t ensemble_lds_bytes(const EnsArgs& a, int mode);
 hipError_t launch_ensemble_stats(const double* s_spec, int n_real, int n_k, int nq, double* kmean,
                                  double* mean, double* stdev, hipStream_t s);
 
-// The FP64 matrix form of the phase sums (bgw_matrix.hip): workgroups of
-// kMxMfmaWaves matrix wavefronts and kMxProdWaves producer wavefronts, one per
-// tile of `ni` items of one (realisation, reference bubble) and block of
-// kMxMfmaWaves * 16 * omega_tiles omegas.  tiles: (n_tiles, ni) item indices,
-// -1 padding; every item of a tile has the same t_a.
-constexpr int kMxMfmaWaves = 8;
-constexpr int kMxProdWaves = 4;
-constexpr int kMatrixThreads = 64 * (kMxMfmaWaves + kMxProdWaves);
+// The FP64 matrix form of the phase sums (bgw_matrix.hip), over tiles of `ni`
+// items of one (realisation, reference bubble); every item of a tile has the
+// same t_a.  launch_matrix runs tiles [tile0, tile0 + n_tiles) of the arrays
+// below: their time tables, their weight rows (vbuf, rows of 16 col_tiles
+// doubles from tinfo[tile].kbase to the end of the time grid, ring after
+// ring, at tinfo[tile].voff - voff0), and the matrix products into partial.
+struct MxTileInfo {
+  int64_t voff;           // first double of the tile's weight rows
+  int32_t kbase;          // step of row 0: the first weighted step of the bubble, rounded down to 4
+  int32_t len;            // rows per ring: n_t - kbase rounded up to 4
+};
 struct MatArgs {
   SpecArgs s;             // the phase-sum arguments (no damping; n_t <= 65535)
-  const int32_t* tiles;
+  const int32_t* tiles;   // (n_tiles, ni) item indices, -1 padding
+  const MxTileInfo* tinfo;
   int ni;                 // items per tile, <= matrix_items_per_tile(s.n_sets, col_tiles)
-  int omega_tiles;        // 16-omega tiles per matrix wavefront: 1 or 2
-  int col_tiles;          // 16-column tiles per workgroup: 1 or 2
+  int omega_tiles;        // 16-omega tiles per wavefront of the matrix kernel: 1, 2 or 4
+  int col_tiles;          // 16-column tiles: 1 or 2
+  int tile0;              // first tile of the launch
+  int64_t voff0;          // tinfo offset of vbuf[0]
+  double* vbuf;           // weight rows of the launch's tiles
+  double* tt;             // (n_tiles, n_t, 1 + n_sets n_terms) time tables
+  int32_t* kstart;        // (n_tiles, n_c) first weighted step of each ring
+  int32_t* ctlo;          // (n_tiles, n_c, 2) first weighted step of each column tile
 };
+// n_sets * n_terms of one matrix-form launch
+constexpr int kMaxMatrixTerms = 12;
 int matrix_items_per_tile(int n_sets, int col_tiles);
 size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles);
 hipError_t launch_matrix(const MatArgs& a, int n_tiles, hipStream_t s);

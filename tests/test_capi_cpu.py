@@ -225,7 +225,7 @@ def test_spectrum_and_vjp_validation():
     assert _lib.last_error() == "Invalid resolution: n_t must be >= 2 for integration"
     t = _lib.BgwTileConfig(5, 0, 0, 0)
     assert L.bgw_generalized_bulk_flow_set_tiles(h, ctypes.byref(t)) == _lib.CALCULATION_ERROR
-    assert _lib.last_error() == ("tiles.block_steps must be 0, 8 or 16 (1 or 2 for the FP64 "
+    assert _lib.last_error() == ("tiles.block_steps must be 0, 8 or 16 (1, 2 or 4 for the FP64 "
                                  "matrix form)")
     t = _lib.BgwTileConfig(8, 96, 0, 0)
     assert L.bgw_generalized_bulk_flow_set_tiles(h, ctypes.byref(t)) == _lib.CALCULATION_ERROR
