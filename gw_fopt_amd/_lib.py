@@ -87,6 +87,8 @@ SIGNATURES = {
         _st, [_vp, _dp, _sz, _b, _d, _d, _sz, _sp, _sz, BgwTimeCutoffConfig, _dp, _sz]),
     "bgw_generalized_bulk_flow_compute_c_integral_ex": (
         _st, [_vp, _dp, _sz, _b, _d, _d, _sz, _sp, _sz, BgwTimeCutoffConfig, _st, _dp, _sz]),
+    "bgw_generalized_bulk_flow_compute_c_integral_no_alloc": (
+        _st, [_vp, _dp, _sz, _b, _d, _d, _sz, _sp, _sz, BgwTimeCutoffConfig, _st, _dp, _sz]),
     "bgw_generalized_bulk_flow_first_colliding_bubble": (_st, [_vp, _sz, _ip, _sz]),
     "bgw_generalized_bulk_flow_first_colliding_bubble_ex": (_st, [_vp, _sz, _dp, _ip, _dp, _sz]),
     "bgw_generalized_bulk_flow_delta_tab": (_st, [_vp, _sz, _ip, _sz, _dp, _sz]),

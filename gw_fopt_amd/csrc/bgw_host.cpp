@@ -19,6 +19,7 @@
 #include <functional>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <sstream>
@@ -294,11 +295,34 @@ Cutoff make_cutoff(const BgwTimeCutoffConfig& c) {
 // ---------------------------------------------------------------------------
 // Device state
 // ---------------------------------------------------------------------------
+// Bump allocator over caller-provided device memory (the scratch of the
+// no-allocation entry point): regions 256-byte aligned, never freed.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~size_t(255);
+    if (at > cap || bytes > cap - at)
+      throw CalcError("the request needs more device memory than the " +
+                      std::to_string(cap >> 20) +
+                      " MiB scratch of the no-allocation entry point (" +
+                      std::to_string(at + bytes) + " bytes so far)");
+    used = at + bytes;
+    return base + at;
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  Arena* arena = nullptr;  // set: regions come from the arena, no hipMalloc / hipFree
   void ensure(size_t bytes) {
     if (bytes <= cap && p) return;
+    if (arena) {
+      p = arena->take(bytes > 0 ? bytes : 16);
+      cap = bytes;
+      return;
+    }
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -313,7 +337,7 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !arena) (void)hipFree(p);
   }
 };
 
@@ -380,9 +404,58 @@ struct Engine {
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
   BgwTimings timings{};
+  std::vector<DevBuf*> all_buffers() {
+    return {&d_cand, &d_inv, &d_real, &d_items, &d_order, &d_rot, &d_sin_t, &d_cos_t,
+            &d_cos_p, &d_sin_p, &d_gcos, &d_gsin, &d_ring_c, &d_ring_fac, &d_omega, &d_coef,
+            &d_pows, &d_tau, &d_perm, &d_partial, &d_group_off, &d_group_r, &d_c, &d_s,
+            &d_first, &d_dtab, &d_gradc, &d_gradout, &d_iota, &d_acc, &d_tiles, &d_tinfo,
+            &d_tt, &d_kstart, &d_ctlo, &d_vbuf, &d_units, &d_unit_off, &d_counter, &d_kmean,
+            &d_emean, &d_estd};
+  }
   ~Engine() {
     if (graph) (void)hipGraphExecDestroy(graph);
   }
+};
+
+// For the lifetime of the scope every buffer of E is carved from the arena
+// (the static candidate lists are uploaded again into it); the engine's own
+// allocations are put back afterwards, untouched.  The destructor waits for
+// the engine's streams, so nothing in flight still uses the arena when the
+// call returns, also on an error.
+class ArenaScope {
+ public:
+  ArenaScope(Engine& E, Arena& A) : E_(E), uploaded_(E.static_uploaded) {
+    for (DevBuf* b : E.all_buffers()) {
+      saved_.push_back({b->p, b->cap, b->arena});
+      b->p = nullptr;
+      b->cap = 0;
+      b->arena = &A;
+    }
+    E.static_uploaded = false;
+  }
+  ~ArenaScope() {
+    if (E_.dev.stream) (void)hipStreamSynchronize(E_.dev.stream);
+    if (E_.dev.stream2) (void)hipStreamSynchronize(E_.dev.stream2);
+    const std::vector<DevBuf*> bufs = E_.all_buffers();
+    for (size_t i = 0; i < bufs.size(); ++i) {
+      bufs[i]->p = saved_[i].p;
+      bufs[i]->cap = saved_[i].cap;
+      bufs[i]->arena = saved_[i].arena;
+    }
+    E_.static_uploaded = uploaded_;
+  }
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+
+ private:
+  struct Saved {
+    void* p;
+    size_t cap;
+    Arena* arena;
+  };
+  Engine& E_;
+  bool uploaded_;
+  std::vector<Saved> saved_;
 };
 
 void build_static(Engine& E, const std::vector<Bubbles>& reals, size_t n_threads) {
@@ -1695,6 +1768,34 @@ BgwStatus bgw_generalized_bulk_flow_set_gradient_scaling_params(
   });
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point: every buffer of its calls is carved
+// from this array, one copy per device, held until the process ends.  Host code
+// uses only the address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<char*> g_scratch_ptr;  // by device id
+
+// The device address of `scratch` on device `dev`, which must be current.
+static char* no_alloc_scratch(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0) throw CalcError("no current HIP device");
+  if (g_scratch_ptr.size() <= static_cast<size_t>(dev))
+    g_scratch_ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw CalcError(std::string("hipGetSymbolAddress of the no-allocation scratch failed: ") +
+                      hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 static BgwStatus compute_c_integral_impl(bgw_generalized_bulk_flow* bulk_flow,
                                          const double* w_arr, size_t n_w, bool has_t_begin,
                                          double t_begin, double t_end, size_t n_t,
@@ -1736,6 +1837,37 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integral_ex(
   return compute_c_integral_impl(bulk_flow, w_arr, n_w, has_t_begin, t_begin, t_end, n_t,
                                  selected_bubbles, n_selected_bubbles, time_cutoff,
                                  kernel_mode(precision), out, out_len);
+}
+
+BgwStatus bgw_generalized_bulk_flow_compute_c_integral_no_alloc(
+    bgw_generalized_bulk_flow* bulk_flow, const double* w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t* selected_bubbles,
+    size_t n_selected_bubbles, BgwTimeCutoffConfig time_cutoff, BgwPrecision precision,
+    BgwComplex64* out, size_t out_len) {
+  if (!valid_precision(precision)) {
+    set_last_error("precision is not a BgwPrecision");
+    return BgwStatus_InvalidArgument;
+  }
+  return run_ffi([&] {
+    auto* h = bulk_flow_handle(bulk_flow);
+    const size_t expected = checked_product(2 * h->sc.n_sets, n_w, "out");
+    if (out_len != expected)
+      throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
+                      std::to_string(expected));
+    slice_from_raw(out, out_len, "out");
+    Engine& E = h->engine;
+    E.dev.init();  // makes the handle's device current
+    Arena A;
+    A.base = no_alloc_scratch(E.dev.id);
+    A.cap = kNoAllocScratchBytes;
+    std::vector<double> c;
+    {
+      ArenaScope scope(E, A);
+      c = compute_c(h, w_arr, n_w, has_t_begin, t_begin, t_end, n_t, selected_bubbles,
+                    n_selected_bubbles, time_cutoff, kernel_mode(precision));
+    }
+    if (out_len) std::memcpy(out, c.data(), c.size() * sizeof(double));
+  });
 }
 
 BgwStatus bgw_generalized_bulk_flow_compute_c_tensor(

@@ -223,6 +223,21 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integral_ex(bgw_generalized_bulk_f
                                                           BgwPrecision precision,
                                                           BgwComplex64 *out, size_t out_len);
 
+/* Same as bgw_generalized_bulk_flow_compute_c_integral_ex, but allocates no
+ * device memory: every device buffer of the call (bubbles, candidate lists,
+ * collision delays, partial sums, C) is carved from a 64 MiB scratch array
+ * that the library holds on each device it runs on until the process ends.
+ * A request needing more fails with CalculationError and a message giving the
+ * limit.  Calls that overlap on one device share that scratch and race; the
+ * call returns after its device work has finished.  No reference counterpart:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+BgwStatus bgw_generalized_bulk_flow_compute_c_integral_no_alloc(
+    bgw_generalized_bulk_flow *bulk_flow, const double *w_arr, size_t n_w, bool has_t_begin,
+    double t_begin, double t_end, size_t n_t, const size_t *selected_bubbles,
+    size_t n_selected_bubbles, BgwTimeCutoffConfig time_cutoff, BgwPrecision precision,
+    BgwComplex64 *out, size_t out_len);
+
 /* First colliding bubble of reference bubble a_idx on the (n_cos_thetax,
  * n_phix) grid, int32 encoding -1 None, i Interior(i), -(j+2) Exterior(j).
  * Replaces GeneralizedBulkFlow::compute_first_colliding_bubble
