@@ -68,7 +68,7 @@ def parse():
                    help="only run --steps compute_c_tensor calls (the tensor form C_ij) in this "
                         "form at plain_256's bubbles, grid, times and omegas (for rocprofv3)")
     p.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic counters")
-    p.add_argument("--modes", default="mixed,fp64_blocked,fp64")
+    p.add_argument("--modes", default="mixed,fp64_blocked,fp64_matrix,fp64")
     p.add_argument("--profile-mode", default=None,
                    help="run only this form for --steps steps (the program rocprofv3 profiles)")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -91,37 +91,39 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy_bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
 
 
-def _pmc_run(counter: str, args: list[str], timeout: int = 240) -> dict[str, list[float]]:
-    """Per-kernel-name list of the counter (KiB per dispatch) from one rocprofv3 run."""
+def _pmc_run(counters: list[str], args: list[str], timeout: int = 240) -> dict:
+    """{counter: {kernel name: [value per dispatch]}} from one rocprofv3 run."""
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     out = tempfile.mkdtemp(prefix="bgw_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
-    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+    cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", "run", "--",
            sys.executable, os.path.abspath(__file__), *args]
     try:
         subprocess.run(cmd, check=True, timeout=timeout, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL, cwd=os.environ.get("TMPDIR", "/tmp"))
-        res: dict[str, list[float]] = {}
+        res: dict = {c: {} for c in counters}
         for path in glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
             with open(path) as f:
                 for row in csv.DictReader(f):
-                    if row.get("Counter_Name") == counter:
-                        res.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+                    c = row.get("Counter_Name")
+                    if c in res:
+                        res[c].setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
         return res
     finally:
         shutil.rmtree(out, ignore_errors=True)
 
 
-def _mean_for(res: dict[str, list[float]], needle: str):
-    vals = [v for k, vs in res.items() if needle in k for v in vs]
-    return sum(vals) / len(vals) if vals else None
+def _per_call(res: dict, needles, n_calls: int):
+    """Counter summed over the dispatches of the kernels named by `needles`, per call."""
+    vals = [v for k, vs in res.items() if any(n in k for n in needles) for v in vs]
+    return sum(vals) / n_calls if vals else None
 
 
 def pmc_copy_calibration():
     """FETCH_SIZE and WRITE_SIZE read on a device copy of 1 GiB (the largest
     dispatch of the calibration run)."""
     try:
-        cf = _pmc_run("FETCH_SIZE", ["--calibrate-copy"])
-        cw = _pmc_run("WRITE_SIZE", ["--calibrate-copy"])
+        cf = _pmc_run(["FETCH_SIZE"], ["--calibrate-copy"])["FETCH_SIZE"]
+        cw = _pmc_run(["WRITE_SIZE"], ["--calibrate-copy"])["WRITE_SIZE"]
         copy_f = max((v for vs in cf.values() for v in vs), default=None)
         copy_w = max((v for vs in cw.values() for v in vs), default=None)
         if copy_f is None or copy_w is None or copy_f <= 0 or copy_w <= 0:
@@ -131,40 +133,63 @@ def pmc_copy_calibration():
         return None
 
 
+# kernels of each part of a step, by name
+KERNELS = {
+    "geometry": ("geometry_kernel",),
+    "spectrum": ("spectrum_kernel", "mx_time_kernel", "mx_weights_kernel", "mx_gemm_kernel"),
+    "matrix": ("mx_gemm_kernel",),
+    "weights": ("mx_weights_kernel",),
+}
+PMC_CALLS = 3  # --warmup 1 --steps 2
+
+
 def pmc_flops(mode: str, n_k: int):
-    """Executed FP64 VALU flops per launch of the phase-sum and geometry kernels:
-    SQ_INSTS_VALU_FLOPS_FP64 counts flops per wave instruction (an FMA counts 2),
-    times 64 lanes.  Returns {"spectrum": F, "geometry": F} (None where missing)."""
+    """Executed FP64 flops per call of each part of the step: vector FP64
+    (SQ_INSTS_VALU_FLOPS_FP64 counts flops per wave instruction, an FMA as 2,
+    times 64 lanes) plus matrix-core FP64 (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512).
+    Returns {part: {"valu": F, "mfma": F, "total": F}} (None where missing)."""
+    empty = {k: None for k in KERNELS}
     try:
         prog = ["--profile-mode", mode, "--steps", "2", "--warmup", "1", "--n-k", str(n_k)]
-        res = _pmc_run("SQ_INSTS_VALU_FLOPS_FP64", prog)
+        res = _pmc_run(["SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_MFMA_MOPS_F64"], prog)
         out = {}
-        for key, needle in (("spectrum", "spectrum_kernel"), ("geometry", "geometry_kernel")):
-            v = _mean_for(res, needle)
-            out[key] = None if v is None else 64.0 * v
+        for part, needles in KERNELS.items():
+            v = _per_call(res["SQ_INSTS_VALU_FLOPS_FP64"], needles, PMC_CALLS)
+            mf = _per_call(res["SQ_INSTS_VALU_MFMA_MOPS_F64"], needles, PMC_CALLS)
+            if v is None:
+                out[part] = None
+                continue
+            valu, mfma = 64.0 * v, 512.0 * (mf or 0.0)
+            out[part] = {"valu": valu, "mfma": mfma, "total": valu + mfma}
         return out
     except Exception:
-        return {"spectrum": None, "geometry": None}
+        return empty
 
 
-def pmc_traffic(mode: str, n_k: int, calibration):
-    """HBM bytes per phase-sum launch: FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch) of the profiled kernel, each scaled by bytes per unit measured the
-    same way on a 1 GiB device copy."""
+def pmc_traffic(mode: str, n_k: int, calibration, parts=("spectrum",)):
+    """HBM bytes per call of the named parts: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by bytes per unit measured the same way on a 1 GiB
+    device copy."""
     if calibration is None:
         return None
     copy_f, copy_w = calibration
     try:
         prog = ["--profile-mode", mode, "--steps", "2", "--warmup", "1", "--n-k", str(n_k)]
-        fetch = _mean_for(_pmc_run("FETCH_SIZE", prog), "spectrum_kernel")
-        write = _mean_for(_pmc_run("WRITE_SIZE", prog), "spectrum_kernel")
-        if fetch is None or write is None:
-            return None
-        f_scale = (1 << 30) / copy_f   # bytes per counted KiB-unit, reads
+        fetch = _pmc_run(["FETCH_SIZE"], prog)["FETCH_SIZE"]
+        write = _pmc_run(["WRITE_SIZE"], prog)["WRITE_SIZE"]
+        f_scale = (1 << 30) / copy_f   # bytes per counted unit, reads
         w_scale = (1 << 30) / copy_w   # writes
-        return {"fetch_bytes": fetch * f_scale, "write_bytes": write * w_scale,
-                "bytes": fetch * f_scale + write * w_scale,
-                "fetch_scale": f_scale / 1024.0, "write_scale": w_scale / 1024.0}
+        out = {"fetch_scale_bytes_per_kib": f_scale / 1024.0,
+               "write_scale_bytes_per_kib": w_scale / 1024.0}
+        for part in parts:
+            f = _per_call(fetch, KERNELS[part], PMC_CALLS)
+            w = _per_call(write, KERNELS[part], PMC_CALLS)
+            if f is None or w is None:
+                out[part] = None
+            else:
+                out[part] = {"fetch_bytes": f * f_scale, "write_bytes": w * w_scale,
+                             "bytes": f * f_scale + w * w_scale}
+        return out
     except Exception:
         return None
 
@@ -199,7 +224,7 @@ def run_mode(bf, wl, mode, steps, warmup):
     import numpy as np
     for _ in range(warmup):
         bf.compute_spectrum(wl.w_arr, wl.k_hat, t_end=wl.t_end, n_t=wl.n_t, precision=mode)
-    dev, geo, spec, red = [], [], [], []
+    dev, geo, spec, red, mat = [], [], [], [], []
     _sync()
     t0 = time.perf_counter()
     S = None
@@ -210,11 +235,12 @@ def run_mode(bf, wl, mode, steps, warmup):
         geo.append(tm["geometry_ms"])
         spec.append(tm["spectrum_ms"])
         red.append(tm["reduce_ms"])
+        mat.append(tm["matrix_ms"])
     _sync()
     wall = (time.perf_counter() - t0) / steps
     return S, {"device_ms": float(np.mean(dev)), "geometry_ms": float(np.mean(geo)),
                "spectrum_ms": float(np.mean(spec)), "reduce_ms": float(np.mean(red)),
-               "wall_ms": wall * 1e3}
+               "matrix_ms": float(np.mean(mat)), "wall_ms": wall * 1e3}
 
 
 def cpu_baseline(wl, n_k_sample=2):
@@ -433,9 +459,10 @@ def main():
     traffic, executed = {}, {}
     if not args.no_pmc:
         cal = pmc_copy_calibration()
-        for m in ("fp64_blocked", "mixed"):
-            traffic[m] = pmc_traffic(m, args.n_k, cal)
-        for m in ("fp64_blocked", "fp64"):
+        traffic["fp64_matrix"] = pmc_traffic("fp64_matrix", args.n_k, cal,
+                                             parts=("matrix", "weights", "spectrum", "geometry"))
+        traffic["fp64_blocked"] = pmc_traffic("fp64_blocked", args.n_k, cal)
+        for m in ("fp64_matrix", "fp64_blocked", "fp64"):
             executed[m] = pmc_flops(m, args.n_k)
 
     from gw_fopt_amd import many_bubbles as mb
@@ -467,11 +494,27 @@ def main():
     spec_flops = flops["spectrum"]
     algorithmic_rate = spec_flops / (r["spectrum_ms"] * 1e-3) / 1e12
     ex = executed.get(main_mode) or {}
-    ex_spec, ex_geo = ex.get("spectrum"), ex.get("geometry")
-    achieved = None if ex_spec is None else ex_spec / (r["spectrum_ms"] * 1e-3) / 1e12
-    dtypes = {"fp64": "f64", "fp64_blocked": "f64",
+
+    def rate(part, ms):
+        e = ex.get(part)
+        return None if e is None or not ms else e["total"] / (ms * 1e-3) / 1e12
+
+    dtypes = {"fp64": "f64", "fp64_blocked": "f64", "fp64_matrix": "f64",
               "mixed": "f32 sincos/accumulation, f64 masks and argument reduction"}
-    tr = traffic.get(main_mode)
+    tr = (traffic.get(main_mode) or {})
+    if main_mode == "fp64_matrix":
+        # the dominant kernel: the matrix-core products (HIP events around its launches)
+        kern, part, ms, bound = "mx_gemm_kernel (FP64 matrix form)", "matrix", r["matrix_ms"], "mfma"
+        basis = ("executed FP64 flops per launch: matrix cores (rocprofv3 "
+                 "SQ_INSTS_VALU_MFMA_MOPS_F64 x 512) + vector (SQ_INSTS_VALU_FLOPS_FP64 x 64 "
+                 "lanes), same workload, / the kernel's HIP-event time; peak: MI355X dense "
+                 "FP64 matrix")
+    else:
+        kern, part, ms, bound = f"spectrum_kernel ({main_mode} form)", "spectrum", r["spectrum_ms"], "valu"
+        basis = ("executed FP64 VALU flops per launch (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 "
+                 "lanes, same workload) / the kernel's HIP-event time; peak: MI355X vector FP64")
+    achieved = rate(part, ms)
+    ktr = tr.get(part) if isinstance(tr, dict) else None
     line = {
         "metric": "bulk-flow (omega,k-hat) points/s: mixed vs FP64 GPU @ <=1e-6 rel err; grad/s",
         "value": value,
@@ -495,21 +538,19 @@ def main():
                                    "the bubbles resident in HBM; excludes the host-side call "
                                    "set-up and the copies of omegas in and spectra out "
                                    "(wall_points_per_s includes them)"},
-        "roofline": {"bound": "valu", "kernel": f"spectrum_kernel ({main_mode} form)",
+        "roofline": {"bound": bound, "kernel": kern,
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": None if achieved is None else achieved / FP64_PEAK_TFLOPS,
-                     "achieved_basis": "executed FP64 VALU flops per launch "
-                                       "(rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes, same "
-                                       "workload) / the kernel's event time",
-                     "traffic": None if tr is None else tr["bytes"],
-                     "executed_fp64_flops_per_launch": ex_spec, "launch_ms": r["spectrum_ms"],
-                     "hbm_GBs": None if tr is None else tr["bytes"] / (r["spectrum_ms"] * 1e-3) / 1e9,
-                     "algorithmic_flops_per_launch": spec_flops,
+                     "traffic": None if ktr is None else ktr["bytes"],
+                     "achieved_basis": basis,
+                     "executed_flops_per_launch": ex.get(part), "launch_ms": ms,
+                     "hbm_GBs": None if ktr is None or not ms else ktr["bytes"] / (ms * 1e-3) / 1e9,
+                     "algorithmic_flops_per_step": spec_flops,
                      "algorithmic_rate_tflops": algorithmic_rate,
                      "algorithmic_note": "the workload's formula count 10 S_A + 10 S_C + 8 E_B "
-                                         "(one sincos per phase term) over the kernel time; the "
-                                         "factorised forms execute fewer operations, so this "
-                                         "rate is not a fraction of any peak"},
+                                         "over the phase-sum time; it assumes one sincos per "
+                                         "phase term, which none of the factorised forms "
+                                         "executes, so this rate is not a fraction of any peak"},
         "modes": {m: {"points_per_s": points / (v["device_ms"] * 1e-3),
                       "wall_points_per_s": points / (v["wall_ms"] * 1e-3),
                       "err_vs_fp64": errs.get(m), **v}
@@ -519,11 +560,27 @@ def main():
     if "fp64" in results:
         base = line["modes"]["fp64"]["points_per_s"]
         line["speedup_vs_fp64"] = {m: v["points_per_s"] / base for m, v in line["modes"].items()}
-    geo_ex = None if ex_geo is None else ex_geo / (r["geometry_ms"] * 1e-3) / 1e12
+    # executed rates of every measured form's phase sums (all their kernels) and geometry
+    line["executed"] = {}
+    for m, exm in executed.items():
+        if m not in results or not exm:
+            continue
+        rm = results[m]
+        ent = {}
+        for p_, ms_ in (("spectrum", rm["spectrum_ms"]), ("geometry", rm["geometry_ms"]),
+                        ("matrix", rm["matrix_ms"])):
+            e = exm.get(p_)
+            if e is None or not ms_:
+                continue
+            t = e["total"] / (ms_ * 1e-3) / 1e12
+            ent[p_] = {"flops_per_call": e, "ms": ms_, "tflops": t, "frac": t / FP64_PEAK_TFLOPS}
+        line["executed"][m] = ent
+    geo_ex = (ex.get("geometry") or {}).get("total") if ex.get("geometry") else None
+    geo_rate = None if geo_ex is None else geo_ex / (r["geometry_ms"] * 1e-3) / 1e12
     line["geometry"] = {"kernel": "geometry_kernel", "launch_ms": r["geometry_ms"],
-                        "executed_fp64_flops_per_launch": ex_geo,
-                        "executed_tflops": geo_ex,
-                        "frac": None if geo_ex is None else geo_ex / FP64_PEAK_TFLOPS,
+                        "executed_fp64_flops_per_launch": geo_ex,
+                        "executed_tflops": geo_rate,
+                        "frac": None if geo_rate is None else geo_rate / FP64_PEAK_TFLOPS,
                         "algorithmic_flops_per_launch": flops["geometry"],
                         "algorithmic_rate_tflops": flops["geometry"] / (r["geometry_ms"] * 1e-3) / 1e12,
                         "algorithmic_note": "10 flops per (bubble, direction, candidate) test; "

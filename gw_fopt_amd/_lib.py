@@ -40,7 +40,7 @@ class BgwTimings(ctypes.Structure):
     _fields_ = [("geometry_ms", ctypes.c_double), ("spectrum_ms", ctypes.c_double),
                 ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("n_items", ctypes.c_size_t), ("n_terms", ctypes.c_size_t),
-                ("n_directions", ctypes.c_size_t)]
+                ("n_directions", ctypes.c_size_t), ("matrix_ms", ctypes.c_double)]
 
 
 class BgwTileConfig(ctypes.Structure):

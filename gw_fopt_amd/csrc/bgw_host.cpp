@@ -354,6 +354,8 @@ struct Device {
   hipStream_t stream2 = nullptr;  // second branch of the pipelined ensemble graph
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t dep[4] = {nullptr, nullptr, nullptr, nullptr};  // graph dependencies (no timing)
+  static constexpr int kTimedProducts = 16;  // matrix-core launches timed per call
+  hipEvent_t mx_ev[2 * kTimedProducts] = {};
   void init() {
     if (stream) {
       HIPCHECK(hipSetDevice(id));
@@ -370,12 +372,16 @@ struct Device {
       if (!e) HIPCHECK(hipEventCreate(&e));
     for (auto& e : dep)
       if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : mx_ev)
+      if (!e) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   }
   ~Device() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : dep)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : mx_ev)
       if (e) (void)hipEventDestroy(e);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
@@ -404,6 +410,8 @@ struct Engine {
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
   BgwTimings timings{};
+  int mx_timed = 0;         // matrix-core launches of this call with events
+  bool mx_untimed = false;  // a launch beyond Device::kTimedProducts
   std::vector<DevBuf*> all_buffers() {
     return {&d_cand, &d_inv, &d_real, &d_items, &d_order, &d_rot, &d_sin_t, &d_cos_t,
             &d_cos_p, &d_sin_p, &d_gcos, &d_gsin, &d_ring_c, &d_ring_fac, &d_omega, &d_coef,
@@ -815,7 +823,17 @@ void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw
   for (const auto& b : batches) {
     ma.tile0 = static_cast<int>(b.first);
     ma.voff0 = info[b.first].voff;
-    HIPCHECK(bgw::launch_matrix(ma, static_cast<int>(b.second - b.first), st));
+    const int nt = static_cast<int>(b.second - b.first);
+    HIPCHECK(bgw::launch_matrix_weights(ma, nt, st));
+    const int slot = E.mx_timed;
+    if (slot < Device::kTimedProducts) HIPCHECK(hipEventRecord(E.dev.mx_ev[2 * slot], st));
+    HIPCHECK(bgw::launch_matrix_products(ma, nt, st));
+    if (slot < Device::kTimedProducts) {
+      HIPCHECK(hipEventRecord(E.dev.mx_ev[2 * slot + 1], st));
+      ++E.mx_timed;
+    } else {
+      E.mx_untimed = true;
+    }
   }
 }
 
@@ -840,6 +858,8 @@ Result run(Engine& E, const Request& q) {
   const bgw::SpecArgs& sa = Lc.sa;
   const bgw::RedArgs& ra = Lc.ra;
 
+  E.mx_timed = 0;
+  E.mx_untimed = false;
   HIPCHECK(hipEventRecord(E.dev.ev[0], st));
   HIPCHECK(bgw::launch_geometry(ga, static_cast<int>(n_items), st));
   HIPCHECK(hipEventRecord(E.dev.ev[1], st));
@@ -890,6 +910,10 @@ Result run(Engine& E, const Request& q) {
   E.timings.n_items = n_items;
   E.timings.n_directions = n_items * n_dir;
   E.timings.n_terms = n_items * res.n_c * q.n_t * q.n_w;
+  E.timings.matrix_ms = 0.0;
+  for (int k = 0; k < E.mx_timed; ++k)
+    E.timings.matrix_ms += elapsed_ms(E.dev.mx_ev[2 * k], E.dev.mx_ev[2 * k + 1]);
+  if (E.mx_untimed) E.timings.matrix_ms = std::nan("");
   return out;
 }
 
@@ -939,6 +963,7 @@ std::vector<double> run_adjoint(Engine& E, const Request& q, const double* grad_
   E.timings.geometry_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
   E.timings.spectrum_ms = elapsed_ms(E.dev.ev[1], E.dev.ev[2]);
   E.timings.reduce_ms = 0.0;
+  E.timings.matrix_ms = 0.0;
   E.timings.total_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[2]);
   E.timings.n_items = n_items;
   E.timings.n_directions = n_items * q.res->n_c * q.res->n_phi;
@@ -997,6 +1022,7 @@ std::vector<double> run_integrand(Engine& E, const Request& q) {
   E.timings.geometry_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
   E.timings.spectrum_ms = elapsed_ms(E.dev.ev[1], E.dev.ev[2]);
   E.timings.reduce_ms = 0.0;
+  E.timings.matrix_ms = 0.0;
   E.timings.total_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[2]);
   E.timings.n_items = n_items;
   E.timings.n_directions = n_items * q.res->n_c * q.res->n_phi;
@@ -1366,6 +1392,7 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   E.timings.geometry_ms = 0.0;
   E.timings.spectrum_ms = out.ms;
   E.timings.reduce_ms = 0.0;
+  E.timings.matrix_ms = 0.0;
   E.timings.total_ms = out.ms;
   E.timings.n_items = Lc.n_items;
   E.timings.n_directions = Lc.n_items * n_dir;

@@ -112,7 +112,8 @@ hipError_t launch_ensemble_stats(const double* s_spec, int n_real, int n_k, int 
 
 // The FP64 matrix form of the phase sums (bgw_matrix.hip), over tiles of `ni`
 // items of one (realisation, reference bubble); every item of a tile has the
-// same t_a.  launch_matrix runs tiles [tile0, tile0 + n_tiles) of the arrays
+// same t_a.  launch_matrix_weights and launch_matrix_products run tiles
+// [tile0, tile0 + n_tiles) of the arrays
 // below: their time tables, their weight rows (vbuf, rows of 16 col_tiles
 // doubles from tinfo[tile].kbase to the end of the time grid, ring after
 // ring, at tinfo[tile].voff - voff0), and the matrix products into partial.
@@ -139,7 +140,9 @@ struct MatArgs {
 constexpr int kMaxMatrixTerms = 12;
 int matrix_items_per_tile(int n_sets, int col_tiles);
 size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles);
-hipError_t launch_matrix(const MatArgs& a, int n_tiles, hipStream_t s);
+// the time tables and weight rows of the launch's tiles, then their matrix products
+hipError_t launch_matrix_weights(const MatArgs& a, int n_tiles, hipStream_t s);
+hipError_t launch_matrix_products(const MatArgs& a, int n_tiles, hipStream_t s);
 
 // Default launch shape of the phase sums (0 in SpecArgs::mixed_block /
 // threads): time steps per factorised block, 8 for the FP64 blocked form and

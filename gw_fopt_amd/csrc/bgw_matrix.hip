@@ -564,20 +564,33 @@ size_t matrix_lds_bytes(const SpecArgs& a, int ni, int /*col_tiles*/) {
          static_cast<size_t>(ni) * L.KS * sizeof(uint16_t);
 }
 
-hipError_t launch_matrix(const MatArgs& a, int n_tiles, hipStream_t s) {
+namespace {
+
+bool matrix_args_ok(const MatArgs& a) {
+  if (a.omega_tiles != 1 && a.omega_tiles != 2 && a.omega_tiles != 4) return false;
+  if (a.col_tiles != 1 && a.col_tiles != 2) return false;
+  if (a.ni < 1 || a.ni > matrix_items_per_tile(a.s.n_sets, a.col_tiles)) return false;
+  if (a.s.n_t > 65535 || a.s.has_damping) return false;
+  if (a.s.n_sets * a.s.n_terms > kMaxMatrixTerms) return false;
+  return matrix_lds_bytes(a.s, a.ni, a.col_tiles) <= 160 * 1024;
+}
+
+}  // namespace
+
+hipError_t launch_matrix_weights(const MatArgs& a, int n_tiles, hipStream_t s) {
   if (n_tiles == 0) return hipSuccess;
-  if (a.omega_tiles != 1 && a.omega_tiles != 2 && a.omega_tiles != 4) return hipErrorInvalidValue;
-  if (a.col_tiles != 1 && a.col_tiles != 2) return hipErrorInvalidValue;
-  if (a.ni < 1 || a.ni > matrix_items_per_tile(a.s.n_sets, a.col_tiles)) return hipErrorInvalidValue;
-  if (a.s.n_t > 65535 || a.s.has_damping) return hipErrorInvalidValue;
-  if (a.s.n_sets * a.s.n_terms > kMaxMatrixTerms) return hipErrorInvalidValue;
+  if (!matrix_args_ok(a)) return hipErrorInvalidValue;
   const size_t lds = matrix_lds_bytes(a.s, a.ni, a.col_tiles);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(mx_time_kernel, dim3(n_tiles), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = a.col_tiles == 1 ? launch_weights_t<1>(a, n_tiles, lds, s) : launch_weights_t<2>(a, n_tiles, lds, s);
-  if (e != hipSuccess) return e;
+  return a.col_tiles == 1 ? launch_weights_t<1>(a, n_tiles, lds, s)
+                          : launch_weights_t<2>(a, n_tiles, lds, s);
+}
+
+hipError_t launch_matrix_products(const MatArgs& a, int n_tiles, hipStream_t s) {
+  if (n_tiles == 0) return hipSuccess;
+  if (!matrix_args_ok(a)) return hipErrorInvalidValue;
   const int w_per_block = 4 * 16 * a.omega_tiles;
   const dim3 grid(n_tiles, (a.s.n_w + w_per_block - 1) / w_per_block);
   if (a.col_tiles == 1) {

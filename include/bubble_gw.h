@@ -93,6 +93,8 @@ typedef struct BgwTimings {
     size_t n_items;      /* (realisation, k-hat, bubble) work items */
     size_t n_terms;      /* (item, ring, time step, omega) phase terms evaluated */
     size_t n_directions; /* (item, surface direction) collision searches */
+    double matrix_ms;    /* FP64 matrix form: the matrix-core kernel's launches (within
+                            spectrum_ms); NAN when not every launch was timed; 0 otherwise */
 } BgwTimings;
 
 /* A batch of independent nucleation realisations (uneven bubble counts). */
