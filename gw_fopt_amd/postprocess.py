@@ -22,7 +22,80 @@ from scipy.interpolate import CubicSpline
 from scipy.optimize import least_squares
 
 __all__ = ["spectrum_from_c", "DataGwBulkFlow", "general_broken_power_law",
-           "asymptotic_broken_power_law", "fit_broken_power_law"]
+           "asymptotic_broken_power_law", "fit_broken_power_law", "sample", "sample_arr",
+           "filter_dataframe"]
+
+
+def sample(start, stop, n_sample, n_grid, n_iter, sample_type, base=10.0) -> np.ndarray:
+    """The reference's ``gw_fopt.bubble_gw.utils.sample`` (rust/py-bubble-gw/src/py_utils.rs:42-110,
+    rust/bubble-gw/src/utils/sample.rs:174-315): n_sample + 1 points from start to stop,
+    evenly spaced in the transformed space of sample_type ("uniform"/"linear": x, "log":
+    log_base x, "exp": base^x of an even grid); with n_iter > 0 only the points that
+    refinement level n_iter adds (n_grid - 1 per cell of the level n_iter - 1 grid)."""
+    kind = str(sample_type).lower()
+    if kind not in ("uniform", "linear", "log", "exp"):
+        raise ValueError(f"Invalid sample_type: {sample_type}. Expected 'uniform', 'linear', "
+                         "'log', or 'exp'.")
+    base = float(base)
+    if kind in ("log", "exp") and base <= 0.0:
+        kname = "logarithmic" if kind == "log" else "exponential"
+        raise ValueError(f"`base` must be greater than 0 for {kname} sampling")
+    start, stop = float(start), float(stop)
+    if start >= stop:
+        raise ValueError(f"`start` ({start}) must be less than `stop` ({stop})")
+    if np.isnan(start) or np.isnan(stop):
+        raise ValueError("Invalid parameter start or stop: NaN")
+    if kind in ("log", "exp") and (start <= 0.0 or stop <= 0.0):
+        raise ValueError(f"`start` ({start}) must be less than `stop` ({stop})")
+    n_sample, n_grid, n_iter = int(n_sample), int(n_grid), int(n_iter)
+    if n_sample == 0:
+        return np.array([start])
+    if n_iter > 0 and n_grid < 2:
+        raise ValueError(f"Invalid parameter n_grid: {float(n_grid)}")
+    if kind == "log":
+        fwd, inv = (lambda x: np.log(x) / np.log(base)), (lambda x: base ** x)
+    elif kind == "exp":
+        fwd, inv = (lambda x: x), (lambda x: base ** x)
+    else:
+        fwd, inv = (lambda x: x), (lambda x: x)
+    a, b = fwd(start), fwd(stop)
+    if n_iter == 0:
+        t = np.arange(n_sample + 1, dtype=np.float64) / n_sample
+    else:
+        factor = float(n_grid) ** n_iter
+        sub = float(n_grid) ** (n_iter - 1)
+        cells = np.arange(n_sample * int(sub), dtype=np.float64)
+        fine = np.arange(1, n_grid, dtype=np.float64)
+        t = ((fine[None, :] / factor + cells[:, None] / sub) / n_sample).reshape(-1)
+    return np.asarray(inv(a + t * (b - a)), dtype=np.float64)
+
+
+def sample_arr(start, stop, n_sample, n_grid, n_iter, sample_type, base=10.0) -> np.ndarray:
+    """Levels 0..n_iter of ``sample`` concatenated (sample.rs:234-249)."""
+    return np.concatenate([sample(start, stop, n_sample, n_grid, it, sample_type, base)
+                           for it in range(int(n_iter) + 1)])
+
+
+def filter_dataframe(df, filter_dict):
+    """Rows of df that meet every criterion: numeric columns against a number within
+    rtol 1e-5 / atol 1e-8, anything else by equality
+    (src/gw_fopt/bubble_dynamics/utils/filter_dataframe.py:69-131, the cases the bulk-flow
+    post-processing uses)."""
+    import pandas as pd
+    mask = pd.Series(True, index=df.index)
+    for col, crit in filter_dict.items():
+        if col not in df.columns:
+            raise ValueError(f"Column '{col}' not found.")
+        series = df[col]
+        if pd.api.types.is_numeric_dtype(series) and isinstance(crit, (int, float, np.number)):
+            mask &= np.isclose(series, crit, rtol=1e-5, atol=1e-8)
+        else:
+            mask &= series == crit
+    return df[mask]
+
+
+def _is_dataframe(x) -> bool:
+    return hasattr(x, "iterrows") and hasattr(x, "columns")
 
 
 def spectrum_from_c(w, c_plus, c_cross) -> np.ndarray:
@@ -34,14 +107,28 @@ def spectrum_from_c(w, c_plus, c_cross) -> np.ndarray:
 class DataGwBulkFlow:
     """Bulk-flow C integrals on a cos(theta_k) grid, and the spectra built from them.
 
+    Two constructors: the reference's ``DataGwBulkFlow(df, d0=1.0)`` over its
+    DataFrame (columns d, cos_theta_k, w_sample, powers_sets_sample, c_bulkflow,
+    c_envelope, _pset_id; rows taken in the frame's order, as the reference
+    iterates them, against the sorted cos grid), and the array form below.
+
     ``c_bulkflow``: (n_cos, 2, n_powers, n_w) or (n_cos, 2, n_w) complex, [C+, Cx]
     for each power of ``powers_sets_sample``; ``c_envelope``: (n_cos, 2, n_w) or
     None.  Rows are kept in ascending cos(theta_k) order (the reference sorts the
     cos grid, data_gw_bulkflow.py:22, and integrates over it).
     """
 
-    def __init__(self, w_arr, cos_thetak_arr, c_bulkflow, powers_sets_sample=None,
+    def __init__(self, w_arr, cos_thetak_arr=None, c_bulkflow=None, powers_sets_sample=None,
                  c_envelope=None, d0: float = 1.0):
+        self.df = None
+        if _is_dataframe(w_arr):
+            # the reference's DataGwBulkFlow(df, d0=1.0) (data_gw_bulkflow.py:10-24)
+            if c_bulkflow is not None or powers_sets_sample is not None or c_envelope is not None:
+                raise TypeError("DataGwBulkFlow(df, d0) takes no arrays")
+            self._init_frame(w_arr, d0 if cos_thetak_arr is None else cos_thetak_arr)
+            return
+        if cos_thetak_arr is None or c_bulkflow is None:
+            raise TypeError("DataGwBulkFlow takes (df, d0) or (w_arr, cos_thetak_arr, c_bulkflow, ...)")
         self.d0 = float(d0)
         self.w_arr = np.asarray(w_arr, dtype=np.float64)
         cos = np.asarray(cos_thetak_arr, dtype=np.float64)
@@ -63,6 +150,56 @@ class DataGwBulkFlow:
             if e.shape != (len(cos), 2, len(self.w_arr)):
                 raise ValueError("c_envelope must be (n_cos, 2, n_w)")
             self.c_envelope = e[order]
+
+    def _init_frame(self, df, d0):
+        """Rows with column d equal to d0; cos(theta_k) sorted; omegas from the first
+        row's w_sample arguments of ``sample`` (label 0, as the reference indexes)."""
+        self.d0 = d0
+        self.df = filter_dataframe(df, {"d": d0})
+        self.cos_thetak_arr = np.sort(self.df.cos_theta_k.values)
+        self.w_arr = sample(*self.df["w_sample"][0])
+        self.c_bulkflow = None
+        self.c_envelope = None
+        self.powers_sets_sample = None
+
+    def _frame_spectra(self, xi_powers, xi_coefficients) -> np.ndarray:
+        """Per row of the frame, in its order (data_gw_bulkflow.py:65-115): C+ and Cx of
+        the combination from c_envelope (None) or the row's c_bulkflow set of power
+        xi + 3 among sample(*powers_sets_sample)."""
+        w = self.w_arr
+        out = []
+        for _, row in self.df.iterrows():
+            cp = np.zeros_like(w, dtype=complex)
+            cx = np.zeros_like(w, dtype=complex)
+            available = None
+            if any(p is not None for p in xi_powers):
+                available = list(sample(*row["powers_sets_sample"]))
+            for p, coeff in zip(xi_powers, xi_coefficients):
+                if p is None:
+                    cm = np.asarray(row["c_envelope"])
+                    if cm.ndim != 2 or cm.shape[0] != 2:
+                        raise ValueError(f"c_envelope for _pset_id {row['_pset_id']} must be a 2D "
+                                         "array with shape[0]=2")
+                    cp += coeff * cm[0, :]
+                    cx += coeff * cm[1, :]
+                    continue
+                power = p + 3
+                if available is None:
+                    raise ValueError("powers_sets_sample required for non-None powers")
+                if power not in available:
+                    raise ValueError(f"Power {power} not in powers_sets_sample for _pset_id "
+                                     f"{row['_pset_id']}")
+                k = available.index(power)
+                cm = np.asarray(row["c_bulkflow"])
+                if cm.ndim == 2 and cm.shape[0] == 2:
+                    cm = cm[:, np.newaxis, :]
+                elif cm.ndim != 3 or cm.shape[0] != 2:
+                    raise ValueError(f"c_bulkflow for _pset_id {row['_pset_id']} must be a 2D or "
+                                     "3D array with shape[0]=2")
+                cp += coeff * cm[0, k, :]
+                cx += coeff * cm[1, k, :]
+            out.append(spectrum_from_c(w, cp, cx))
+        return np.array(out)
 
     @classmethod
     def from_bulk_flow(cls, bulk_flow, w_arr, cos_thetak_arr, powers, *, t_end, n_t, t_begin=None,
@@ -96,6 +233,8 @@ class DataGwBulkFlow:
             raise ValueError("xi_coefficients must sum to 1")
         if not all(p is None or isinstance(p, (int, float, np.integer, np.floating)) for p in xi_powers):
             raise ValueError("xi_powers must contain numbers or None")
+        if self.df is not None:
+            return self._frame_spectra(xi_powers, xi_coefficients)
         n_cos, n_w = self.c_bulkflow.shape[0], self.c_bulkflow.shape[3]
         c_plus = np.zeros((n_cos, n_w), dtype=np.complex128)
         c_cross = np.zeros((n_cos, n_w), dtype=np.complex128)

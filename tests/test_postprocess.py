@@ -114,3 +114,99 @@ def test_from_bulk_flow_matches_oracle():
     cx = 0.6 * Co[:, 1, 0] + 0.4 * Co[:, 1, 2]
     want = 8 * np.pi * w**3 * (abs(cp) ** 2 + abs(cx) ** 2)
     assert np.max(np.abs(s - want) / np.abs(want)) <= 1e-10
+
+
+# ---------------------------------------------------------------------------
+# the reference's sample() and DataGwBulkFlow(df, d0)
+# ---------------------------------------------------------------------------
+def test_sample_known_answers():
+    """Expected values of rust/bubble-gw/tests/test_sample.rs:75-160 and the examples
+    of rust/py-bubble-gw/src/py_utils.rs:33-37."""
+    np.testing.assert_allclose(pp.sample(0.0, 10.0, 4, 2, 0, "linear"), [0, 2.5, 5, 7.5, 10], atol=1e-12)
+    np.testing.assert_allclose(pp.sample(0.0, 10.0, 4, 2, 0, "uniform"), [0, 2.5, 5, 7.5, 10], atol=1e-12)
+    np.testing.assert_allclose(pp.sample(1.0, 100.0, 4, 2, 0, "log"),
+                               [1.0, 3.16227766017, 10.0, 31.6227766017, 100.0], rtol=1e-10)
+    np.testing.assert_allclose(pp.sample(0.001, 2.0, 4, 2, 0, "exp"),
+                               [1.0023052380778996, 3.1677434383924457, 10.011519555381687,
+                                31.640985375587913, 100.0], rtol=1e-8)
+    np.testing.assert_allclose(pp.sample_arr(0.0, 12.0, 3, 2, 2, "linear"),
+                               [0, 4, 8, 12, 2, 6, 10, 1, 3, 5, 7, 9, 11], atol=1e-12)
+    np.testing.assert_allclose(pp.sample(0, 5, 5, 2, 0, "uniform"), [0, 1, 2, 3, 4, 5])
+    np.testing.assert_allclose(pp.sample(0, 5, 5, 2, 1, "uniform"), [0.5, 1.5, 2.5, 3.5, 4.5])
+    np.testing.assert_allclose(pp.sample(0, 5, 5, 2, 2, "uniform"), np.arange(0.25, 5, 0.5))
+    # n_grid = 3: two new points per cell of the previous level
+    np.testing.assert_allclose(pp.sample(0, 3, 3, 3, 1, "linear"), [1 / 3, 2 / 3, 4 / 3, 5 / 3, 7 / 3, 8 / 3])
+    with pytest.raises(ValueError, match="Invalid parameter n_grid"):
+        pp.sample(0.0, 10.0, 4, 1, 1, "uniform")
+    with pytest.raises(ValueError, match=r"`start` \(2.0\) must be less than `stop` \(1.0\)"):
+        pp.sample(2.0, 1.0, 4, 2, 0, "linear")
+    with pytest.raises(ValueError, match="Invalid sample_type"):
+        pp.sample(0.0, 1.0, 4, 2, 0, "cubic")
+
+
+def _frame(seed=3):
+    """A frame of the reference's layout: two nucleation scales d, four cos(theta_k)
+    rows each (unsorted), omegas sample(0.5, 20, 6, 2, 0, 'log'), powers
+    sample(3, 5, 2, 2, 0, 'linear') = [3, 4, 5]."""
+    import pandas as pd
+    rng = np.random.default_rng(seed)
+    rows = []
+    n_w = 7
+    for d in (1.0, 2.0):
+        for q, cos in enumerate([0.5, -1.0, 1.0, 0.0]):
+            c = rng.normal(size=(2, 3, n_w)) + 1j * rng.normal(size=(2, 3, n_w))
+            env = rng.normal(size=(2, n_w)) + 1j * rng.normal(size=(2, n_w))
+            rows.append({"_pset_id": f"d{d}-{q}", "d": d, "cos_theta_k": cos,
+                         "w_sample": (0.5, 20.0, 6, 2, 0, "log"),
+                         "powers_sets_sample": (3.0, 5.0, 2, 2, 0, "linear"),
+                         "c_bulkflow": c, "c_envelope": env})
+    return pd.DataFrame(rows)
+
+
+def test_data_frame_constructor():
+    df = _frame()
+    d = pp.DataGwBulkFlow(df, 1.0)
+    assert d.d0 == 1.0 and len(d.df) == 4
+    np.testing.assert_array_equal(d.cos_thetak_arr, [-1.0, 0.0, 0.5, 1.0])
+    np.testing.assert_allclose(d.w_arr, np.geomspace(0.5, 20.0, 7), rtol=1e-13)
+    # S per row, in the frame's order: xi = 2 -> power 5 (index 2), None -> envelope
+    s = d.de_dlogw_dcosthetak([2, None], [0.25, 0.75])
+    sub = df[df.d == 1.0]
+    for r, (_, row) in enumerate(sub.iterrows()):
+        cp = 0.25 * row["c_bulkflow"][0, 2] + 0.75 * row["c_envelope"][0]
+        cx = 0.25 * row["c_bulkflow"][1, 2] + 0.75 * row["c_envelope"][1]
+        np.testing.assert_allclose(s[r], 8 * np.pi * d.w_arr**3 * (abs(cp) ** 2 + abs(cx) ** 2),
+                                   rtol=1e-14)
+    # a frame already in ascending cos order gives the array form's results
+    srt = sub.sort_values("cos_theta_k").reset_index(drop=True)
+    d2 = pp.DataGwBulkFlow(srt, d0=1.0)
+    arr = pp.DataGwBulkFlow(d2.w_arr, srt.cos_theta_k.values,
+                            np.stack(list(srt.c_bulkflow)), powers_sets_sample=[3.0, 4.0, 5.0],
+                            c_envelope=np.stack(list(srt.c_envelope)))
+    for xi, co in (([2, None], [0.25, 0.75]), ([1], [1.0]), ([0, 2], [0.5, 0.5])):
+        np.testing.assert_allclose(d2.de_dlogw(xi, co), arr.de_dlogw(xi, co), rtol=1e-14)
+        sp, sa = d2.de_dlogw_interp(xi, co, d=3.0), arr.de_dlogw_interp(xi, co, d=3.0)
+        np.testing.assert_allclose(sp(d2.w_arr / 3.0), sa(d2.w_arr / 3.0), rtol=1e-13)
+    # d filter within rtol 1e-5: 1.00001 selects the d = 1 rows
+    assert len(pp.DataGwBulkFlow(df, 1.00001).df) == 4
+    # the omegas come from the row labelled 0 (data_gw_bulkflow.py:23 indexes w_sample by
+    # label), so a selection without that label fails as the reference's does
+    with pytest.raises(KeyError):
+        pp.DataGwBulkFlow(df, 2.0)
+    d3 = pp.DataGwBulkFlow(df[df.d == 2.0].reset_index(drop=True), 2.0)
+    assert len(d3.df) == 4 and d3.d0 == 2.0
+
+
+def test_data_frame_errors():
+    df = _frame()
+    d = pp.DataGwBulkFlow(df, d0=1.0)
+    with pytest.raises(ValueError, match=r"Power 9 not in powers_sets_sample for _pset_id d1.0-0"):
+        d.de_dlogw_dcosthetak([6], [1.0])
+    with pytest.raises(ValueError, match="xi_coefficients must sum to 1"):
+        d.de_dlogw([2], [0.5])
+    with pytest.raises(ValueError, match="Column 'd' not found."):
+        pp.DataGwBulkFlow(df.drop(columns=["d"]), 1.0)
+    bad = df.copy()
+    bad.at[0, "c_envelope"] = np.zeros(7)
+    with pytest.raises(ValueError, match="c_envelope for _pset_id d1.0-0 must be a 2D array"):
+        pp.DataGwBulkFlow(bad, 1.0).de_dlogw_dcosthetak([None], [1.0])
