@@ -1465,9 +1465,10 @@ struct bgw_generalized_bulk_flow {
   Bubbles bubbles;
   Resolution res;
   Scaling sc;
-  size_t num_threads = 0;
+  size_t num_threads = 0;  // 0: the hardware's thread count
   BgwTileConfig tiles{0, 0, 0, 0};
   Engine engine;
+  bool built = false;      // candidate lists of `bubbles` built into `engine`
 };
 
 struct bgw_ensemble {
@@ -1500,6 +1501,19 @@ void apply_tiles(const BgwTileConfig& t, Request& q) {
 bgw_generalized_bulk_flow* bulk_flow_handle(bgw_generalized_bulk_flow* h) {
   if (!h) throw CalcError(null_error("bulk_flow"));
   return h;
+}
+
+size_t host_threads(size_t requested);
+
+// The handle's engine, its per-bubble candidate lists built on first use on
+// host_threads(num_threads) threads (the reference's thread pool of
+// set_num_threads, lib.rs:432-446).
+Engine& built_engine(bgw_generalized_bulk_flow* h) {
+  if (!h->built) {
+    build_static(h->engine, {h->bubbles}, host_threads(h->num_threads));
+    h->built = true;
+  }
+  return h->engine;
 }
 
 const Resolution& need_resolution(const Resolution& r) {
@@ -1579,7 +1593,7 @@ std::vector<double> compute_c(bgw_generalized_bulk_flow* h, const double* w_arr,
   q.want_s = false;
   if (surface) q.skip_uncollided = false;  // components such as xz may be small
   apply_tiles(h->tiles, q);
-  Result r = run(h->engine, q);
+  Result r = run(built_engine(h), q);
   return r.c;
 }
 
@@ -1610,7 +1624,7 @@ std::vector<std::vector<double>> compute_c_surfaces(bgw_generalized_bulk_flow* h
   q.want_s = false;
   q.skip_uncollided = false;  // components such as xz may be small
   apply_tiles(h->tiles, q);
-  Result r = run(h->engine, q);
+  Result r = run(built_engine(h), q);
   const size_t blk = 2 * n_sets * n_w * 2;  // one group's C
   std::vector<std::vector<double>> out;
   for (size_t k = 0; k < surfaces.size(); ++k)
@@ -1654,7 +1668,7 @@ void integrand_into(bgw_generalized_bulk_flow* h, const double* w_arr, size_t n_
     q.t_end = {t_end};
     q.bubbles_of_real = {ids};
     q.skip_uncollided = false;
-    res_t = run_integrand(h->engine, q);
+    res_t = run_integrand(built_engine(h), q);
   }
   for (int pm = 0; pm < 2; ++pm) {
     if (dst[pm] < 0) continue;
@@ -1762,7 +1776,6 @@ BgwStatus bgw_generalized_bulk_flow_empty_lattice_new(bgw_generalized_bulk_flow*
       throw CalcError("too many bubbles");
     auto h = std::make_unique<bgw_generalized_bulk_flow>();
     h->bubbles = make_bubbles(bubbles_interior, n_interior, bubbles_exterior, n_exterior);
-    build_static(h->engine, {h->bubbles}, host_threads(0));
     *out = h.release();
   });
 }
@@ -1882,7 +1895,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_c_integral_no_alloc(
       throw CalcError("out_len=" + std::to_string(out_len) + " does not match expected length " +
                       std::to_string(expected));
     slice_from_raw(out, out_len, "out");
-    Engine& E = h->engine;
+    Engine& E = built_engine(h);
     E.dev.init();  // makes the handle's device current
     Arena A;
     A.base = no_alloc_scratch(E.dev.id);
@@ -1955,7 +1968,7 @@ BgwStatus bgw_generalized_bulk_flow_first_colliding_bubble(bgw_generalized_bulk_
     slice_from_raw(out, out_len, "out");
     std::vector<int32_t> first;
     std::vector<double> dtab;
-    run_masks(h->engine, res, static_cast<int32_t>(a_idx), nullptr, first, dtab);
+    run_masks(built_engine(h), res, static_cast<int32_t>(a_idx), nullptr, first, dtab);
     std::memcpy(out, first.data(), n * sizeof(int32_t));
   });
 }
@@ -1981,7 +1994,7 @@ BgwStatus bgw_generalized_bulk_flow_first_colliding_bubble_ex(bgw_generalized_bu
     }
     std::vector<int32_t> first;
     std::vector<double> dtab;
-    run_masks(h->engine, res, static_cast<int32_t>(a_idx), k_hat, first, dtab);
+    run_masks(built_engine(h), res, static_cast<int32_t>(a_idx), k_hat, first, dtab);
     if (first_out) std::memcpy(first_out, first.data(), n * sizeof(int32_t));
     if (delta_tab_out) std::memcpy(delta_tab_out, dtab.data(), n * sizeof(double));
   });
@@ -2156,7 +2169,7 @@ BgwStatus bgw_generalized_bulk_flow_compute_spectrum(
       q.bubbles_of_real = {ids};
       q.mode = kernel_mode(precision);
       apply_tiles(h->tiles, q);
-      Result r = run(h->engine, q);
+      Result r = run(built_engine(h), q);
       S = std::move(r.s);
       C = std::move(r.c);
     }
@@ -2213,7 +2226,7 @@ BgwStatus bgw_generalized_bulk_flow_spectrum_vjp(
       std::vector<int32_t> ids(h->bubbles.n_int);
       std::iota(ids.begin(), ids.end(), 0);
       q.bubbles_of_real = {ids};
-      g = run_adjoint(h->engine, q, reinterpret_cast<const double*>(grad_c));
+      g = run_adjoint(built_engine(h), q, reinterpret_cast<const double*>(grad_c));
     }
     std::memcpy(grad_coefficients, g.data(), grad_len * sizeof(double));
     std::memcpy(grad_powers, g.data() + grad_len, grad_len * sizeof(double));

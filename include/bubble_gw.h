@@ -180,7 +180,10 @@ BgwStatus bgw_generalized_bulk_flow_empty_lattice_new(bgw_generalized_bulk_flow 
 void bgw_generalized_bulk_flow_free(bgw_generalized_bulk_flow *bulk_flow);
 
 /* Reference: bgw_generalized_bulk_flow_set_num_threads (lib.rs:432-446).
- * num_threads sizes the host thread pool used by host-side preparation. */
+ * num_threads (> 0) sizes the host thread pool of the per-configuration
+ * preparation -- each bubble's partners sorted by a collision-delay bound --
+ * which runs at the handle's first compute call; the default is the
+ * hardware's thread count.  Everything per k-hat runs on the GPU. */
 BgwStatus bgw_generalized_bulk_flow_set_num_threads(bgw_generalized_bulk_flow *bulk_flow,
                                                     size_t num_threads);
 

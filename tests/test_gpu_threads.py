@@ -74,3 +74,19 @@ def test_separate_handles_on_threads():
     for j in range(len(runs)):
         for r in range(reps):
             assert got[j][r].tobytes() == alone[j].tobytes(), (j, r)
+
+
+def test_set_num_threads_sizes_the_preparation_only():
+    """set_num_threads(n) sets the host threads of the per-configuration candidate
+    build (run at the first compute call); the results do not depend on it."""
+    inter, ext = _bubbles(7)
+    out = []
+    for n in (None, 1, 3):
+        bf = mb.GeneralizedBulkFlow(mb.LatticeBubbles(mb.EmptyLattice(), inter, ext))
+        if n is not None:
+            bf.set_num_threads(n)
+        bf.set_resolution(24, 48)
+        bf.set_gradient_scaling_params([[1.0]], [[5.0]])
+        out.append(bf.compute_spectrum(W, K, t_end=float(inter[:, 0].max()) + 0.6, n_t=400,
+                                       precision="fp64")[0])
+    assert out[0].tobytes() == out[1].tobytes() == out[2].tobytes()
