@@ -200,6 +200,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   __shared__ int tlo_s[16];
   __shared__ double rf_s[16];
   __shared__ double tau0_s[16];
+  __shared__ int ks_s;
   const SpecArgs& g = m.s;
   const int tile = m.tile0 + static_cast<int>(blockIdx.x);
   const int i = blockIdx.y;
@@ -345,12 +346,16 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
     const int64_t ti = static_cast<int64_t>(tile) * g.n_c + i;
     m.kstart[ti] = ks;
     for (int c = 0; c < CT; ++c) m.ctlo[ti * 2 + c] = cl[c];
+    ks_s = ks;
   }
-  // (4) the ring's weight rows
+  __syncthreads();
+  // (4) the ring's weight rows from the first one the products read (the ring's
+  // first weighted step rounded down to 4); the rows before it stay unwritten
   const int col0 = (tid & 15) * CT;   // this lane's columns col0 + [0, CT)
   const int jc = col0 / (2 * nsets);  // their item
-  const int RP = (info.len + 15) / 16;
-  const int r0 = (tid >> 4) * RP;
+  const int live0 = ks_s < n_t ? (ks_s & ~3) - info.kbase : info.len;
+  const int RP = (info.len - live0 + 15) / 16;
+  const int r0 = live0 + (tid >> 4) * RP;
   const int r1 = min(info.len, r0 + RP);
   double* vout = m.vbuf + (info.voff - m.voff0) + static_cast<int64_t>(i) * info.len * NCOL;
   const bool live = jc < ni && item_s[jc] >= 0;

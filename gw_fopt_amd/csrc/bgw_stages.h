@@ -351,31 +351,39 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       }
 
       // Pass 1: minimum collision delay over the candidates (dot > 1e-10,
-      // delay > 0), ties to the lowest bubble index.  The list is sorted by a
-      // lower bound `key` of the delay: once key * (1 - 1e-7) > best for every
-      // direction of the wave, no later candidate can reach or tie a best.
-      // dsq > best * 2 dot * (1 + 1e-12) proves dsq / (2 dot) > best (the
+      // delay > 0), ties to the lowest bubble index, and the smallest delay of
+      // any other candidate with dot > 0 (`second`).  The list is sorted by a
+      // lower bound `key` of the delay: the walk goes on while key <= best *
+      // (1 + 1e-4) (1 + 1e-6) for some direction of the wave; later candidates
+      // can neither reach nor tie a best (key (1 - 1e-7) > best) and are beyond
+      // what the verification below has to see (key > best (1 + 1e-4)).
+      // dsq > lim * 2 dot * (1 + 1e-12) proves dsq / (2 dot) > lim (the
       // relative slack exceeds the rounding of both sides), so the division
-      // runs only when some direction has a candidate that can reach its best.
-      double best[D], two_dot[D];
+      // runs only when a candidate can lower the best or the second delay.
+      double best[D], second[D], stop_lim[D], div_lim[D];
       int bidx[D], bpos[D];
-      bool cand[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         best[d] = kInf;
+        second[d] = kInf;
+        stop_lim[d] = kInf;
+        div_lim[d] = kInf;
         bidx[d] = 0x7fffffff;
         bpos[d] = -1;
       }
+      // irregular candidates (key -inf) sort first
+      const bool irregular = n_cand > 0 && !(cand_lds[0].flags & kCandRegular);
       for (int s = 0; s < n_cand; ++s) {
         const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
-        bool any_need = false, any_cand = false;
+        bool need[D], cand[D], any_need = false, any_cand = false;
+        double two_dot[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          const bool need = act[d] && !(e.key * (1.0 - 1e-7) > best[d]);
-          any_need = any_need || need;
+          need[d] = act[d] && !(e.key > stop_lim[d]);
+          any_need = any_need || need[d];
           const double dot = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1[d], x2[d], x3[d]);
           two_dot[d] = 2.0 * dot;
-          cand[d] = need && !(dot <= 1e-10) && !(e.dsq > best[d] * two_dot[d] * (1.0 + 1e-12));
+          cand[d] = need[d] && dot > 0.0 && !(e.dsq > div_lim[d] * two_dot[d]);
           any_cand = any_cand || cand[d];
         }
         if (!wave_any(any_need)) break;
@@ -386,11 +394,17 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
               const double dtv = e.dsq / two_dot[d];
               if (dtv != dtv) {
                 fb[d] = true;
-              } else if (dtv > 0.0 && (dtv < best[d] || (dtv == best[d] && e.idx < bidx[d]))) {
+              } else if (two_dot[d] > 2e-10 && dtv > 0.0 &&
+                         (dtv < best[d] || (dtv == best[d] && e.idx < bidx[d]))) {
+                second[d] = fmin(second[d], best[d]);
                 best[d] = dtv;
                 bidx[d] = e.idx;
                 bpos[d] = s;
+              } else {
+                second[d] = fmin(second[d], dtv);
               }
+              stop_lim[d] = best[d] * ((1.0 + 1e-4) * (1.0 + 1e-6));
+              div_lim[d] = fmax(best[d], second[d]) * (1.0 + 1e-12);
             }
           }
         }
@@ -398,18 +412,23 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
 
       // Pass 2: verify each minimum against every other bubble c with the
       // reference's cross-multiplied test.  For a regular, non-grazing
-      // candidate, bubbles with key > best * (1 + 1e-4) provably pass, so the
-      // wave again walks a prefix of the list.
+      // minimum, a regular bubble c whose delay along the direction exceeds
+      // best * (1 + 1e-4), or that does not reach the direction (dot <= 0),
+      // provably passes: so bubbles with key > best * (1 + 1e-4) are never
+      // tested, and when pass 1 saw no other candidate within that margin
+      // (second > best * (1 + 1e-4)) and the list holds no irregular bubble,
+      // nothing needs testing at all.  The wave walks a prefix of the list
+      // only when one of its directions needs it.
       bool p2[D], ok[D];
       RotCand b0[D];
       double thr[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        p2[d] = act[d] && !fb[d] && bpos[d] >= 0;
+        const bool minimum = act[d] && !fb[d] && bpos[d] >= 0;
         ok[d] = true;
         thr[d] = kInf;
         b0[d] = RotCand{};
-        if (p2[d]) {
+        if (minimum) {
           b0[d] = fetch_cand(bpos[d], cand_lds, n_lds, list, R, identity);
           const double scale =
               fabs(b0[d].d0) + sqrt(b0[d].d1 * b0[d].d1 + b0[d].d2 * b0[d].d2 + b0[d].d3 * b0[d].d3);
@@ -417,8 +436,12 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
           const bool grazing = !(best[d] <= 1e3 * scale);
           if (regular && !grazing) thr[d] = best[d] * (1.0 + 1e-4);
         }
+        p2[d] = minimum && (irregular || !(second[d] > thr[d]));
       }
-      for (int s = 0; s < n_cand; ++s) {
+      bool any_p2 = false;
+#pragma unroll
+      for (int d = 0; d < D; ++d) any_p2 = any_p2 || p2[d];
+      for (int s = 0; wave_any(any_p2) && s < n_cand; ++s) {
         const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
         bool need[D], any_need = false;
 #pragma unroll
@@ -440,7 +463,7 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
         if (fb[d]) {
           result = reference_scan(list, inv, it.a, ri.n_int, ri.n_tot, R, identity, x1[d], x2[d],
                                   x3[d]);
-        } else if (p2[d]) {
+        } else if (act[d] && bpos[d] >= 0) {
           result = bidx[d];
         }
         double tau = kInf;
