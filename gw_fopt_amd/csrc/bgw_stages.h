@@ -24,7 +24,8 @@ constexpr double kInf = __builtin_inf();
 // A candidate in the k-hat frame, as staged in LDS.
 struct alignas(16) RotCand {
   double d0, d1, d2, d3;  // rotated delta
-  double dsq;             // stored delta_squared (unrotated)
+  double hdsq;            // half the stored delta_squared (unrotated): the delay
+                          //   dsq / (2 dot) is hdsq / dot, the same double
   double m;               // scalar(rotated delta, rotated delta)
   double key;
   int32_t idx, flags;
@@ -43,7 +44,7 @@ __device__ __forceinline__ RotCand rotate_entry(const CandEntry& e, const double
     o.d2 = (R[3] * e.d[1] + R[4] * e.d[2]) + R[5] * e.d[3];
     o.d3 = (R[6] * e.d[1] + R[7] * e.d[2]) + R[8] * e.d[3];
   }
-  o.dsq = e.dsq;
+  o.hdsq = 0.5 * e.dsq;
   o.m = lscalar(o.d0, o.d1, o.d2, o.d3, o.d0, o.d1, o.d2, o.d3);
   o.key = e.key;
   o.idx = e.idx;
@@ -84,7 +85,7 @@ __device__ int reference_scan(const CandEntry* list, const int32_t* inv, int a, 
     RotCand eb = rotate_entry(list[inv[b]], R, identity);
     double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1, x2, x3);
     if (dot <= 1e-10) continue;
-    double dtv = eb.dsq / (2.0 * dot);
+    double dtv = eb.hdsq / dot;
     if (dtv <= 0.0 || dtv >= best) continue;
     bool first = true;
     for (int c = 0; c < n_tot; ++c) {
@@ -103,7 +104,8 @@ __device__ int reference_scan(const CandEntry* list, const int32_t* inv, int a, 
   return bidx;
 }
 
-__device__ __forceinline__ bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
+// The predicate's ballot stays a scalar mask (no per-lane materialisation).
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
 }  // namespace
 
@@ -357,9 +359,9 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       // (1 + 1e-4) (1 + 1e-6) for some direction of the wave; later candidates
       // can neither reach nor tie a best (key (1 - 1e-7) > best) and are beyond
       // what the verification below has to see (key > best (1 + 1e-4)).
-      // dsq > lim * 2 dot * (1 + 1e-12) proves dsq / (2 dot) > lim (the
-      // relative slack exceeds the rounding of both sides), so the division
-      // runs only when a candidate can lower the best or the second delay.
+      // hdsq > lim (1 + 1e-12) dot proves hdsq / dot > lim (the relative slack
+      // exceeds the rounding of both sides), so the division runs only when a
+      // candidate can lower the best or the second delay.
       double best[D], second[D], stop_lim[D], div_lim[D];
       int bidx[D], bpos[D];
 #pragma unroll
@@ -373,28 +375,28 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       }
       // irregular candidates (key -inf) sort first
       const bool irregular = n_cand > 0 && !(cand_lds[0].flags & kCandRegular);
-      for (int s = 0; s < n_cand; ++s) {
-        const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
+      // Two copies of the walk: staged candidates from LDS, the rest from HBM
+      // (one loop over both would read the fields through generic pointers).
+      auto step1 = [&](const RotCand& e, int s) -> bool {
         bool need[D], cand[D], any_need = false, any_cand = false;
-        double two_dot[D];
+        double dot[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           need[d] = act[d] && !(e.key > stop_lim[d]);
           any_need = any_need || need[d];
-          const double dot = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1[d], x2[d], x3[d]);
-          two_dot[d] = 2.0 * dot;
-          cand[d] = need[d] && dot > 0.0 && !(e.dsq > div_lim[d] * two_dot[d]);
+          dot[d] = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1[d], x2[d], x3[d]);
+          cand[d] = need[d] && dot[d] > 0.0 && !(e.hdsq > div_lim[d] * dot[d]);
           any_cand = any_cand || cand[d];
         }
-        if (!wave_any(any_need)) break;
+        if (!wave_any(any_need)) return false;
         if (wave_any(any_cand)) {
 #pragma unroll
           for (int d = 0; d < D; ++d) {
             if (cand[d]) {
-              const double dtv = e.dsq / two_dot[d];
+              const double dtv = e.hdsq / dot[d];
               if (dtv != dtv) {
                 fb[d] = true;
-              } else if (two_dot[d] > 2e-10 && dtv > 0.0 &&
+              } else if (dot[d] > 1e-10 && dtv > 0.0 &&
                          (dtv < best[d] || (dtv == best[d] && e.idx < bidx[d]))) {
                 second[d] = fmin(second[d], best[d]);
                 best[d] = dtv;
@@ -408,6 +410,16 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
             }
           }
         }
+        return true;
+      };
+      {
+        int s = 0;
+        bool more = true;
+        for (; more && s < n_lds; ++s) {
+          const RotCand e = cand_lds[s];
+          more = step1(e, s);
+        }
+        for (; more && s < n_cand; ++s) more = step1(rotate_entry(list[s], R, identity), s);
       }
 
       // Pass 2: verify each minimum against every other bubble c with the
@@ -441,19 +453,28 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       bool any_p2 = false;
 #pragma unroll
       for (int d = 0; d < D; ++d) any_p2 = any_p2 || p2[d];
-      for (int s = 0; wave_any(any_p2) && s < n_cand; ++s) {
-        const RotCand e = fetch_cand(s, cand_lds, n_lds, list, R, identity);
+      auto step2 = [&](const RotCand& e, int s) -> bool {
         bool need[D], any_need = false;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           need[d] = p2[d] && ok[d] && !(e.key > thr[d]);
           any_need = any_need || need[d];
         }
-        if (!wave_any(any_need)) break;
+        if (!wave_any(any_need)) return false;
 #pragma unroll
         for (int d = 0; d < D; ++d)
           if (need[d] && s != bpos[d] && !collision_point_ok(b0[d], e, x1[d], x2[d], x3[d]))
             ok[d] = false;
+        return true;
+      };
+      if (wave_any(any_p2)) {
+        int s = 0;
+        bool more = true;
+        for (; more && s < n_lds; ++s) {
+          const RotCand e = cand_lds[s];
+          more = step2(e, s);
+        }
+        for (; more && s < n_cand; ++s) more = step2(rotate_entry(list[s], R, identity), s);
       }
 
 #pragma unroll
@@ -474,7 +495,7 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
             const RotCand eb =
                 fb[d] ? fetch_cand(inv[result], cand_lds, n_lds, list, R, identity) : b0[d];
             const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1[d], x2[d], x3[d]);
-            dtab = fabs(dot) < 1e-10 ? 0.0 : eb.dsq / (2.0 * dot);
+            dtab = fabs(dot) < 1e-10 ? 0.0 : eb.hdsq / dot;
             if (dtab > 0.0) tau = dtab;  // AlreadyCollided needs delta_tab > 0 (:405)
           }
           const int64_t dir = static_cast<int64_t>(ring[d]) * n_phi + jd[d];
