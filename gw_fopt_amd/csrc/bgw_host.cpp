@@ -818,7 +818,7 @@ void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw
     batches.emplace_back(t0, t1);
     t0 = t1;
   }
-  E.d_vbuf.ensure(max_batch * sizeof(double));
+  E.d_vbuf.ensure((max_batch + static_cast<size_t>(bgw::kMatrixRowPad) * 32) * sizeof(double));
   ma.vbuf = E.d_vbuf.as<double>();
   for (const auto& b : batches) {
     ma.tile0 = static_cast<int>(b.first);

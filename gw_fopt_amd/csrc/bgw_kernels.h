@@ -138,6 +138,9 @@ struct MatArgs {
 };
 // n_sets * n_terms of one matrix-form launch
 constexpr int kMaxMatrixTerms = 12;
+// rows of padding (of 32 doubles) after the weight rows of a launch: the
+// products kernel prefetches up to 28 rows past a ring's last row
+constexpr int kMatrixRowPad = 32;
 int matrix_items_per_tile(int n_sets, int col_tiles);
 size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles);
 // the time tables and weight rows of the launch's tiles, then their matrix products

@@ -30,10 +30,14 @@
 //    :588-664, 775-873).  Latency-bound work, run at full occupancy.
 //  * mx_gemm_kernel, one workgroup per (tile, block of omegas): each wavefront
 //    streams the weight rows of every ring as the B operand (loaded four
-//    matrix steps ahead) and generates the A operand, one (omega, step)
-//    element per lane advanced four steps at a time by an FP64 complex product
-//    with exp(i w 4 dt (1 - c_i)) from an exact FP64 sincos anchor every 64
-//    steps; no LDS and no barriers.
+//    matrix steps ahead, from a pointer with immediate offsets) and generates
+//    the A operand, one (omega, step) element per lane advanced four steps at
+//    a time by an FP64 complex product with exp(i w 4 dt (1 - c_i)), restarted
+//    every 64 steps from an anchor that itself advances by exp(i w 64 dt (1 -
+//    c_i)) from the ring's exact FP64 phase; no LDS and no barriers.  FP64
+//    matrix instructions share the vector unit's FP64 datapath (they never
+//    co-execute), so every vector instruction here costs matrix time: the
+//    loop keeps them to the phase products.
 // Operand layouts of v_mfma_f64_16x16x4_f64 (measured on the device): A[m][k]
 // in lane m + 16 k, B[k][n] in lane n + 16 k, D[m][n] in element m / 4 of lane
 // n + 16 (m % 4).  The accumulators stay in VGPRs (this file is compiled with
@@ -49,8 +53,10 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// matrix steps (of 4 time steps) from one exact anchor of the phase factors
-// to the next: at most 15 recurrence products in between
+// matrix steps (of 4 time steps) from one anchor of the phase factors to the
+// next: at most 15 step products in between; the anchors themselves advance
+// by one complex product each from the ring's exact FP64 phase at its first
+// step (a ring of n_t steps carries at most n_t / 64 of them)
 constexpr int kMxAnchorSteps = 16;
 
 __device__ __forceinline__ double dta_at(const RealInfo& ri, int k, double t_a) {
@@ -458,46 +464,61 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
     const int k0 = ks & ~3;
     const double ci = g.ring_c[i];
     const double ai = 1.0 - ci;
-    double sr[OT], si[OT], er[OT], ei[OT];
+    // step factor e^{i w 4 dt (1 - c)}, anchor factor e^{i w 4 kMxAnchorSteps dt (1 - c)},
+    // and the exact phase of matrix step 0
+    double sr[OT], si[OT], fr[OT], fi[OT], er[OT], ei[OT];
+    const double tk0 = ri.t_begin + ri.t_step * static_cast<double>(k0 + kk);
 #pragma unroll
-    for (int o = 0; o < OT; ++o) sincos_f64(om[o] * (4.0 * ri.t_step * ai), &si[o], &sr[o]);
-    // rows of ring i: row r <-> step kbase + r
-    const double* vr = vt + static_cast<int64_t>(i) * info.len * NCOL;
+    for (int o = 0; o < OT; ++o) {
+      sincos_f64(om[o] * (4.0 * ri.t_step * ai), &si[o], &sr[o]);
+      sincos_f64(om[o] * ((4.0 * kMxAnchorSteps) * ri.t_step * ai), &fi[o], &fr[o]);
+      sincos_f64(om[o] * (tk0 * ai + t_a * ci), &ei[o], &er[o]);
+    }
+    double xr[OT], xi[OT];  // the anchor of the current group of kMxAnchorSteps steps
+#pragma unroll
+    for (int o = 0; o < OT; ++o) {
+      xr[o] = er[o];
+      xi[o] = ei[o];
+    }
     const int n_ms = (n_t - k0 + 3) >> 2;
-    // B operands of four matrix steps, the next four loaded while these run
-    double bq[4][CT], bn[4][CT];
-    // Every load is issued (rows past the ring's last one re-read that row and
-    // go unused), so the count of loads in flight is fixed and the wait before
-    // a block's products leaves the next block's loads outstanding.
-    const int last_row = info.len - 1;
-    auto load4 = [&](int ms0, double (&b)[4][CT]) {
+    // this lane's weight row of matrix step 0 (row r <-> step kbase + r); the
+    // rows of a block of four matrix steps sit at immediate offsets from it
+    const double* q =
+        vt + (static_cast<int64_t>(i) * info.len + (k0 - info.kbase) + kk) * NCOL;
+    // B operands of four matrix steps, the next four loaded while these run.
+    // Loads run up to 28 rows past a ring's last row: into the next ring's
+    // rows or the buffer's padding (kMatrixRowPad), never used.
+    auto load4 = [&](const double* qb, double (&b)[4][CT]) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int row = min(last_row, (k0 + 4 * (ms0 + u) - info.kbase) + kk);
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int c = 0; c < CT; ++c) b[u][c] = vr[static_cast<int64_t>(row) * NCOL + c * 16];
-      }
+        for (int c = 0; c < CT; ++c) b[u][c] = qb[u * 4 * NCOL + c * 16];
     };
-    load4(0, bq);
-    for (int ms0 = 0; ms0 < n_ms; ms0 += 4) {
-      load4(ms0 + 4, bn);
+    auto run4 = [&](int ms0, const double (&b)[4][CT]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int ms = ms0 + u;
         if (ms >= n_ms) break;
         const int kb = k0 + 4 * ms;
-        if (ms % kMxAnchorSteps == 0) {
-          const double tk = ri.t_begin + ri.t_step * static_cast<double>(kb + kk);
+        if (ms % kMxAnchorSteps == 0 && ms > 0) {
+          // next anchor = anchor * e^{i w 4 kMxAnchorSteps dt (1 - c)}
 #pragma unroll
-          for (int o = 0; o < OT; ++o) sincos_f64(om[o] * (tk * ai + t_a * ci), &ei[o], &er[o]);
+          for (int o = 0; o < OT; ++o) {
+            const double nr = __builtin_fma(xr[o], fr[o], -(xi[o] * fi[o]));
+            const double nim = __builtin_fma(xr[o], fi[o], xi[o] * fr[o]);
+            xr[o] = nr;
+            xi[o] = nim;
+            er[o] = nr;
+            ei[o] = nim;
+          }
         }
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
           if (kb + 4 <= ct_lo[c]) continue;  // the tile's columns have no weight yet
 #pragma unroll
           for (int o = 0; o < OT; ++o) {
-            acc_re[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[o], bq[u][c], acc_re[o][c], 0, 0, 0);
-            acc_im[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[o], bq[u][c], acc_im[o][c], 0, 0, 0);
+            acc_re[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[o], b[u][c], acc_re[o][c], 0, 0, 0);
+            acc_im[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[o], b[u][c], acc_im[o][c], 0, 0, 0);
           }
         }
 #pragma unroll
@@ -508,10 +529,15 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
           ei[o] = nim;
         }
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int c = 0; c < CT; ++c) bq[u][c] = bn[u][c];
+    };
+    double ba[4][CT], bb[4][CT];
+    load4(q, ba);
+    for (int ms0 = 0; ms0 < n_ms; ms0 += 8) {
+      load4(q + static_cast<int64_t>(ms0 + 4) * 4 * NCOL, bb);
+      run4(ms0, ba);
+      if (ms0 + 4 >= n_ms) break;
+      load4(q + static_cast<int64_t>(ms0 + 8) * 4 * NCOL, ba);
+      run4(ms0 + 4, bb);
     }
   }
 
