@@ -404,7 +404,7 @@ struct Engine {
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
       d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
   // FP64 matrix form: tiles, their time tables and weight rows
-  DevBuf d_tiles, d_tinfo, d_tt, d_kstart, d_ctlo, d_vbuf;
+  DevBuf d_tiles, d_tinfo, d_tt, d_ctlo, d_vbuf;
   // batched ensemble: work units, queue counter, statistics, captured graph
   DevBuf d_units, d_unit_off, d_counter, d_kmean, d_emean, d_estd;
   hipGraphExec_t graph = nullptr;
@@ -417,7 +417,7 @@ struct Engine {
             &d_cos_p, &d_sin_p, &d_gcos, &d_gsin, &d_ring_c, &d_ring_fac, &d_omega, &d_coef,
             &d_pows, &d_tau, &d_perm, &d_partial, &d_group_off, &d_group_r, &d_c, &d_s,
             &d_first, &d_dtab, &d_gradc, &d_gradout, &d_iota, &d_acc, &d_tiles, &d_tinfo,
-            &d_tt, &d_kstart, &d_ctlo, &d_vbuf, &d_units, &d_unit_off, &d_counter, &d_kmean,
+            &d_tt, &d_ctlo, &d_vbuf, &d_units, &d_unit_off, &d_counter, &d_kmean,
             &d_emean, &d_estd};
   }
   ~Engine() {
@@ -796,12 +796,10 @@ void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw
   upload(E.d_tinfo, info, st);
   const size_t nterms = static_cast<size_t>(sb.n_sets) * sb.n_terms;
   E.d_tt.ensure(std::max<size_t>(1, n_tiles * q.n_t * (1 + nterms)) * sizeof(double));
-  E.d_kstart.ensure(std::max<size_t>(1, n_tiles * sb.n_c) * sizeof(int32_t));
   E.d_ctlo.ensure(std::max<size_t>(1, n_tiles * sb.n_c * 2) * sizeof(int32_t));
   ma.tiles = E.d_tiles.as<int32_t>();
   ma.tinfo = E.d_tinfo.as<bgw::MxTileInfo>();
   ma.tt = E.d_tt.as<double>();
-  ma.kstart = E.d_kstart.as<int32_t>();
   ma.ctlo = E.d_ctlo.as<int32_t>();
   // batches of whole tiles within the row budget (a tile larger than the budget alone)
   std::vector<std::pair<size_t, size_t>> batches;

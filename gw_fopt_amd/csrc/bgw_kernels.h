@@ -133,8 +133,8 @@ struct MatArgs {
   int64_t voff0;          // tinfo offset of vbuf[0]
   double* vbuf;           // weight rows of the launch's tiles
   double* tt;             // (n_tiles, n_t, 1 + n_sets n_terms) time tables
-  int32_t* kstart;        // (n_tiles, n_c) first weighted step of each ring
   int32_t* ctlo;          // (n_tiles, n_c, 2) first weighted step of each column tile
+                          //   (written by the weights kernel, one workgroup per column tile)
 };
 // n_sets * n_terms of one matrix-form launch
 constexpr int kMaxMatrixTerms = 12;

@@ -138,10 +138,13 @@ __device__ void wave_scan_strided(double* arr, int n, int stride, bool reverse) 
 // collided weights times tau^p_q summed over the directions < c).  Records of
 // one item are consecutive; items are TS doubles apart (odd, to spread the
 // items over the LDS banks).
-// The tables are built by 16 lanes per item, lane g taking the 2^SLS records
-// [g 2^SLS, (g + 1) 2^SLS); record c sits at off(c) = c W + (c >> SLS), one
-// double of padding per lane's segment, so the 16 lanes of an item touch 16
+// The tables are built by kMxSegLanes lanes per item, lane g taking the 2^SLS
+// records [g 2^SLS, (g + 1) 2^SLS); record c sits at off(c) = c W + (c >> SLS),
+// one double of padding per lane's segment, so the lanes of an item touch
 // different bank pairs when they walk their segments together.
+constexpr int kMxSegLanes = 32;
+constexpr int kMxWeightItems = 256 / kMxSegLanes;  // items of one weights workgroup
+
 struct MxLayout {
   int W, TS, KS, SLS;  // record width, item stride (doubles), collided-step row stride
                        // (uint16), log2 of the records per lane segment
@@ -151,7 +154,7 @@ struct MxLayout {
 __host__ __device__ inline MxLayout mx_layout(int n_phi, int n_sets, int n_terms) {
   MxLayout L;
   L.W = 2 + 2 * n_sets * n_terms;
-  const int sl = (n_phi + 15) / 16;
+  const int sl = (n_phi + kMxSegLanes - 1) / kMxSegLanes;
   L.SLS = 0;
   while ((1 << L.SLS) < sl) ++L.SLS;
   L.TS = L.off(n_phi) + L.W;
@@ -186,31 +189,37 @@ __global__ __launch_bounds__(256) void mx_time_kernel(MatArgs m) {
   }
 }
 
-// One (tile, ring): the tables of its items, then the weight rows k = kbase ..
-// kbase + len - 1 of the ring (16 CT doubles each) into HBM, and the first
-// weighted step of the ring (kstart) and of each column tile (ctlo).
-//  * tables: the 16 lanes t = 16 j + g of item j take the sorted directions
-//    [g 2^SLS, (g + 1) 2^SLS); each sums its segment, a 16-lane scan of the
+// One (tile, ring, column tile ct): the tables of the items whose columns
+// meet columns [16 ct, 16 ct + 16) (at most kMxWeightItems), then those 16
+// columns of the ring's weight rows k = kbase .. kbase + len - 1 into HBM, and
+// the column tile's first weighted step (ctlo).  Splitting a tile's columns
+// over workgroups halves the LDS of each, so twice as many wavefronts keep
+// this latency-bound work in flight.
+//  * tables: the 32 lanes t = 32 j + g of item j take the sorted directions
+//    [g 2^SLS, (g + 1) 2^SLS); each sums its segment, a 32-lane scan of the
 //    segment sums gives every segment's start, and a second walk writes the
 //    records.
-//  * rows: lane t writes columns (t % 16) * CT + [0, CT) of rows (t / 16) RP +
-//    u, so the 16 lanes of a row write it whole; the collided count is found
-//    once by a binary search and carried along the lane's rows.
+//  * rows: lane t writes column 16 ct + t % 16 of rows from the column tile's
+//    first weighted step (rounded down to 4; the products read nothing of the
+//    column tile before it), 16 rows apart; the collided count is found once
+//    by a binary search and carried along the lane's rows.
 // FAST: one set of one term.
-template <int CT, bool FAST>
+template <bool FAST>
 __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   extern __shared__ double smem[];
-  constexpr int NCOL = 16 * CT;
   constexpr int MAXW = FAST ? 4 : 2 + 2 * kMaxMatrixTerms;
-  __shared__ int item_s[16];
-  __shared__ int tlo_s[16];
-  __shared__ double rf_s[16];
-  __shared__ double tau0_s[16];
-  __shared__ int ks_s;
+  constexpr int NJ = kMxWeightItems;
+  __shared__ int item_s[NJ];
+  __shared__ int tlo_s[NJ];
+  __shared__ double rf_s[NJ];
+  __shared__ double tau0_s[NJ];
+  __shared__ int lo_s;
   const SpecArgs& g = m.s;
   const int tile = m.tile0 + static_cast<int>(blockIdx.x);
   const int i = blockIdx.y;
+  const int ct = blockIdx.z;
   const int ni = m.ni;
+  const int ncol = 16 * m.col_tiles;  // doubles per weight row
   const int nsets = FAST ? 1 : g.n_sets;
   const int n_terms = FAST ? 1 : g.n_terms;
   const int nterms = nsets * n_terms;
@@ -220,83 +229,89 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const int W = FAST ? 4 : L.W;
   const int TW = 1 + nterms;
   const MxTileInfo info = m.tinfo[tile];
-  double* tab = smem;                                                              // [ni][TS]
-  uint16_t* kap = reinterpret_cast<uint16_t*>(tab + static_cast<size_t>(ni) * L.TS);  // [ni][KS]
+  // items j0 .. j1 - 1 of the tile have columns in this column tile
+  const int cpi = 2 * nsets;
+  const int j0 = (16 * ct) / cpi;
+  const int j1 = min(ni, (16 * ct + 15) / cpi + 1);
+  const int nj = j1 - j0;
+  const int nj_cap = min(ni, NJ);  // the tables' item slots (matrix_lds_bytes)
+  double* tab = smem;                                                                  // [nj_cap][TS]
+  uint16_t* kap = reinterpret_cast<uint16_t*>(tab + static_cast<size_t>(nj_cap) * L.TS);  // [nj_cap][KS]
   const int tid = threadIdx.x;
-  if (tid < 16) item_s[tid] = tid < ni ? m.tiles[static_cast<int64_t>(tile) * ni + tid] : -1;
+  if (tid < NJ) item_s[tid] = tid < nj ? m.tiles[static_cast<int64_t>(tile) * ni + j0 + tid] : -1;
   __syncthreads();
-  const Item it0 = g.items[item_s[0]];
+  const Item it0 = g.items[m.tiles[static_cast<int64_t>(tile) * ni]];
   const RealInfo ri = g.real[it0.r];
   const double t_a = it0.t_a;
   const double* ttile = m.tt + static_cast<int64_t>(tile) * n_t * TW;  // the tile's time tables
 
   // (1) tables: lane (j, seg) takes directions [seg SL, seg SL + SL) of item j
-  const int j = tid >> 4, seg = tid & 15;
+  const int j = tid / kMxSegLanes, seg = tid % kMxSegLanes;
   const int SL = 1 << L.SLS;
   const int r_lo = min(n_phi, seg * SL), r_hi = min(n_phi, r_lo + SL);
-  const bool jlive = j < ni && item_s[j] >= 0;
-  double* T = tab + static_cast<size_t>(j < ni ? j : 0) * L.TS;
+  const bool jlive = j < nj && item_s[j] >= 0;
+  double* T = tab + static_cast<size_t>(j < nj ? j : 0) * L.TS;
   double sum[MAXW];
 #pragma unroll
   for (int a = 0; a < MAXW; ++a) sum[a] = 0.0;
-  if (j < ni) {
-    const int item = item_s[j];
+  if (j < nj) {
+    const int item = jlive ? item_s[j] : 0;
     const int64_t row = jlive ? (static_cast<int64_t>(item - g.item0) * g.n_c + i) * n_phi : 0;
     const int surf = jlive ? surface_of(g, g.items[item]) : 0;
-    // batches of eight directions: their loads are issued together
-    for (int rb = r_lo; rb < r_hi; rb += 8) {
-    double tb8[8], gp8[8], gx8[8];
-    int pb8[8];
+    // batches of four directions: their loads are issued together
+    for (int rb = r_lo; rb < r_hi; rb += 4) {
+      double tb4[4], gp4[4], gx4[4];
+      int pb4[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const bool ok = jlive && rb + u < r_hi;
-      tb8[u] = ok ? g.tau[row + rb + u] : kInf;
-      pb8[u] = ok ? g.perm[row + rb + u] : 0;
-    }
+      for (int u = 0; u < 4; ++u) {
+        const bool ok = jlive && rb + u < r_hi;
+        tb4[u] = ok ? g.tau[row + rb + u] : kInf;
+        pb4[u] = ok ? g.perm[row + rb + u] : 0;
+      }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const bool ok = jlive && rb + u < r_hi;
-      gp8[u] = ok ? g.gcos[static_cast<int64_t>(surf) * n_phi + pb8[u]] : 0.0;
-      gx8[u] = ok ? g.gsin[static_cast<int64_t>(surf) * n_phi + pb8[u]] : 0.0;
-    }
+      for (int u = 0; u < 4; ++u) {
+        const bool ok = jlive && rb + u < r_hi;
+        gp4[u] = ok ? g.gcos[static_cast<int64_t>(surf) * n_phi + pb4[u]] : 0.0;
+        gx4[u] = ok ? g.gsin[static_cast<int64_t>(surf) * n_phi + pb4[u]] : 0.0;
+      }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int r = rb + u;
-      if (r >= r_hi) break;
-      const double tau = tb8[u], gp = gp8[u], gx = gx8[u];
-      if (r == 0) tau0_s[j] = tau;
-      kap[j * L.KS + r] = static_cast<uint16_t>(first_step_reaching(ri, n_t, t_a, tau));
-      T[L.off(r)] = gp;
-      T[L.off(r) + 1] = gx;
-      sum[0] += gp;
-      sum[1] += gx;
-      const bool fin = tau < kInf;
-      for (int q = 0; q < nterms; ++q) {
-        const double pw = fin ? pow_param(tau, g.pows[q]) : 0.0;
-        const double vp = gp * pw, vx = gx * pw;
-        T[L.off(r + 1) + 2 + 2 * q] = vp;
-        T[L.off(r + 1) + 3 + 2 * q] = vx;
-        sum[2 + 2 * q] += vp;
-        sum[3 + 2 * q] += vx;
+      for (int u = 0; u < 4; ++u) {
+        const int r = rb + u;
+        if (r >= r_hi) break;
+        const double tau = tb4[u], gp = gp4[u], gx = gx4[u];
+        if (r == 0) tau0_s[j] = tau;
+        kap[j * L.KS + r] = static_cast<uint16_t>(first_step_reaching(ri, n_t, t_a, tau));
+        T[L.off(r)] = gp;
+        T[L.off(r) + 1] = gx;
+        sum[0] += gp;
+        sum[1] += gx;
+        const bool fin = tau < kInf;
+        for (int q = 0; q < nterms; ++q) {
+          const double pw = fin ? pow_param(tau, g.pows[q]) : 0.0;
+          const double vp = gp * pw, vx = gx * pw;
+          T[L.off(r + 1) + 2 + 2 * q] = vp;
+          T[L.off(r + 1) + 3 + 2 * q] = vx;
+          sum[2 + 2 * q] += vp;
+          sum[3 + 2 * q] += vx;
+        }
       }
     }
-    }
   }
-  // exclusive prefix of the segment sums over the item's 16 lanes, and their total
+  // exclusive prefix of the segment sums over the item's lanes, and their total
   double excl[MAXW], total[MAXW];
 #pragma unroll
   for (int a = 0; a < MAXW; ++a) {
     if (a >= W) break;
     double v = sum[a];
 #pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-      const double u = __shfl_up(v, off, 16);
+    for (int off = 1; off < kMxSegLanes; off <<= 1) {
+      const double u = __shfl_up(v, off, kMxSegLanes);
       if (seg >= off) v += u;
     }
-    total[a] = __shfl(v, 15, 16);
+    total[a] = __shfl(v, kMxSegLanes - 1, kMxSegLanes);
     excl[a] = v - sum[a];
   }
-  if (j < ni) {
+  if (j < nj) {
     // U: suffix sums (directions >= c), P: prefix sums (directions < c)
     double up = total[0] - excl[0], ux = total[1] - excl[1];
     double pre[MAXW];
@@ -313,13 +328,13 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
         T[L.off(r + 1) + a] = pre[a];
       }
     }
-    if (seg == 15) {
+    if (seg == kMxSegLanes - 1) {
       T[L.off(n_phi)] = 0.0;
       T[L.off(n_phi) + 1] = 0.0;
       for (int a = 2; a < W; ++a) T[a] = 0.0;
     }
   }
-  if (tid < ni) {
+  if (tid < nj) {
     const int item = item_s[tid];
     rf_s[tid] = item >= 0 ? g.ring_fac[static_cast<int64_t>(surface_of(g, g.items[item])) * g.n_c + i]
                           : 0.0;
@@ -329,7 +344,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   // exactly when the ring factor vanishes; a ring whose uncollided sum is
   // rounding noise contributes from its first collision on, and nothing if it
   // never collides
-  if (tid < ni) {
+  if (tid < nj) {
     const double* Tj = tab + static_cast<size_t>(tid) * L.TS;
     const bool noise = fabs(Tj[0]) <= 1e-12 * n_phi && fabs(Tj[1]) <= 1e-12 * n_phi;
     const bool drop_early = g.skip_uncollided && noise && !g.has_damping;
@@ -341,34 +356,27 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   }
   __syncthreads();
   if (tid == 0) {
-    int ks = n_t;
-    int cl[CT];
-    for (int c = 0; c < CT; ++c) cl[c] = n_t;
-    for (int jj = 0; jj < ni; ++jj) {
-      ks = min(ks, tlo_s[jj]);
-      for (int c = 0; c < CT; ++c)
-        if ((jj * 2 * nsets) / 16 <= c && ((jj + 1) * 2 * nsets - 1) / 16 >= c) cl[c] = min(cl[c], tlo_s[jj]);
-    }
-    const int64_t ti = static_cast<int64_t>(tile) * g.n_c + i;
-    m.kstart[ti] = ks;
-    for (int c = 0; c < CT; ++c) m.ctlo[ti * 2 + c] = cl[c];
-    ks_s = ks;
+    int lo = n_t;
+    for (int jj = 0; jj < nj; ++jj) lo = min(lo, tlo_s[jj]);
+    m.ctlo[(static_cast<int64_t>(tile) * g.n_c + i) * 2 + ct] = lo;
+    lo_s = lo;
   }
   __syncthreads();
-  // (4) the ring's weight rows from the first one the products read (the ring's
-  // first weighted step rounded down to 4); the rows before it stay unwritten
-  const int col0 = (tid & 15) * CT;   // this lane's columns col0 + [0, CT)
-  const int jc = col0 / (2 * nsets);  // their item
-  const int live0 = ks_s < n_t ? (ks_s & ~3) - info.kbase : info.len;
+  // (4) this column tile's weight rows from the first one the products read
+  const int col = 16 * ct + (tid & 15);
+  const int jg = col / cpi;           // the column's item in the tile
+  const int jc = jg - j0;             // and in this workgroup
+  const int live0 = lo_s < n_t ? (lo_s & ~3) - info.kbase : info.len;
   const int RP = (info.len - live0 + 15) / 16;
   const int r0 = live0 + (tid >> 4) * RP;
   const int r1 = min(info.len, r0 + RP);
-  double* vout = m.vbuf + (info.voff - m.voff0) + static_cast<int64_t>(i) * info.len * NCOL;
-  const bool live = jc < ni && item_s[jc] >= 0;
+  double* vout = m.vbuf + (info.voff - m.voff0) + static_cast<int64_t>(i) * info.len * ncol;
+  const bool live = jg < ni && jc < nj && item_s[jc] >= 0;
   const int tlo = live ? tlo_s[jc] : n_t;
   const double rf = live ? rf_s[jc] : 0.0;
   const double* Tc = tab + static_cast<size_t>(live ? jc : 0) * L.TS;
   const uint16_t* kr = kap + (live ? jc : 0) * L.KS;
+  const int sc = (col >> 1) % nsets, pm = col & 1;
   int cnt = -1;
   // groups of four rows: their time-table loads are issued together
   for (int rg = r0; rg < r1; rg += 4) {
@@ -385,10 +393,8 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
       const int r = rg + u;
       if (r >= r1) break;
       const int k = info.kbase + r;
-      double v[CT];
-#pragma unroll
-      for (int c = 0; c < CT; ++c) v[c] = 0.0;
-      if (k < n_t && k >= tlo) {
+      double v = 0.0;
+      if (live && k < n_t && k >= tlo) {
         if (cnt < 0) {
           cnt = count_steps_le(kr, n_phi, k);
         } else {
@@ -396,27 +402,19 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
         }
         const double* rec = Tc + L.off(cnt);
         const double base = tb4[u] * rf;
-        const double* tk = ttile + static_cast<int64_t>(k) * TW;
-#pragma unroll
-        for (int c = 0; c < CT; ++c) {
-          const int col = col0 + c;
-          if (col >= ni * 2 * nsets) continue;
-          const int s = (col >> 1) % nsets, pm = col & 1;
-          double b = rec[pm];
-          if (FAST) {
-            b = __builtin_fma(tf4[u], rec[2 + pm], b);
-          } else {
-            for (int kt = 0; kt < n_terms; ++kt) {
-              const int q = s * n_terms + kt;
-              b = __builtin_fma(tk[1 + q], rec[2 + 2 * q + pm], b);
-            }
+        double b = rec[pm];
+        if (FAST) {
+          b = __builtin_fma(tf4[u], rec[2 + pm], b);
+        } else {
+          const double* tk = ttile + static_cast<int64_t>(k) * TW;
+          for (int kt = 0; kt < n_terms; ++kt) {
+            const int q = sc * n_terms + kt;
+            b = __builtin_fma(tk[1 + q], rec[2 + 2 * q + pm], b);
           }
-          v[c] = base * b;
         }
+        v = base * b;
       }
-      double* dst = vout + static_cast<int64_t>(r) * NCOL + col0;
-#pragma unroll
-      for (int c = 0; c < CT; ++c) dst[c] = v[c];
+      vout[static_cast<int64_t>(r) * ncol + col] = v;
     }
   }
 }
@@ -456,11 +454,14 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
   const double* vt = m.vbuf + (info.voff - m.voff0) + (lane & 15);
   for (int i = 0; i < g.n_c; ++i) {
     const int64_t ti = static_cast<int64_t>(tile) * g.n_c + i;
-    const int ks = __builtin_amdgcn_readfirstlane(m.kstart[ti]);
-    if (ks >= n_t) continue;
     int ct_lo[CT];
+    int ks = n_t;
 #pragma unroll
-    for (int c = 0; c < CT; ++c) ct_lo[c] = __builtin_amdgcn_readfirstlane(m.ctlo[ti * 2 + c]);
+    for (int c = 0; c < CT; ++c) {
+      ct_lo[c] = __builtin_amdgcn_readfirstlane(m.ctlo[ti * 2 + c]);
+      ks = min(ks, ct_lo[c]);
+    }
+    if (ks >= n_t) continue;
     const int k0 = ks & ~3;
     const double ci = g.ring_c[i];
     const double ai = 1.0 - ci;
@@ -572,16 +573,15 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
   }
 }
 
-template <int CT>
-hipError_t launch_weights_t(const MatArgs& a, int n_tiles, size_t lds, hipStream_t s) {
+hipError_t launch_weights(const MatArgs& a, int n_tiles, size_t lds, hipStream_t s) {
   const bool fast = a.s.n_sets == 1 && a.s.n_terms == 1;
-  auto k = fast ? mx_weights_kernel<CT, true> : mx_weights_kernel<CT, false>;
+  auto k = fast ? mx_weights_kernel<true> : mx_weights_kernel<false>;
   std::lock_guard<std::mutex> lock(lds_limit_mutex());
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(n_tiles, a.s.n_c), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k, dim3(n_tiles, a.s.n_c, a.col_tiles), dim3(256), lds, s, a);
   return hipGetLastError();
 }
 
@@ -591,8 +591,8 @@ int matrix_items_per_tile(int n_sets, int col_tiles) { return (16 * col_tiles) /
 
 size_t matrix_lds_bytes(const SpecArgs& a, int ni, int /*col_tiles*/) {
   const MxLayout L = mx_layout(a.n_phi, a.n_sets, a.n_terms);
-  return static_cast<size_t>(ni) * L.TS * sizeof(double) +
-         static_cast<size_t>(ni) * L.KS * sizeof(uint16_t);
+  const size_t nj = static_cast<size_t>(ni < kMxWeightItems ? ni : kMxWeightItems);
+  return nj * L.TS * sizeof(double) + nj * L.KS * sizeof(uint16_t);
 }
 
 namespace {
@@ -615,8 +615,7 @@ hipError_t launch_matrix_weights(const MatArgs& a, int n_tiles, hipStream_t s) {
   hipLaunchKernelGGL(mx_time_kernel, dim3(n_tiles), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return a.col_tiles == 1 ? launch_weights_t<1>(a, n_tiles, lds, s)
-                          : launch_weights_t<2>(a, n_tiles, lds, s);
+  return launch_weights(a, n_tiles, lds, s);
 }
 
 hipError_t launch_matrix_products(const MatArgs& a, int n_tiles, hipStream_t s) {
