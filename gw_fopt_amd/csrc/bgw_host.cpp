@@ -525,12 +525,18 @@ struct Request {
   // per-item rows of tau / perm and per-item partial C (false: the fused
   // ensemble kernel, which keeps neither)
   bool item_buffers = true;
+  // run(): also the k-hat mean of S per realisation and the mean and spread of
+  // those over the realisations (an ensemble), inside the timed region
+  bool ensemble_stats = false;
 };
 
 struct Result {
   std::vector<double> c;  // (n_groups, 2, n_sets, n_w, 2)
   std::vector<double> s;  // (n_groups, n_sets, n_w)
   size_t n_groups = 0;
+  // Request::ensemble_stats: (n_real, n_sets, n_w) k-hat means, (n_sets, n_w) mean and
+  // population standard deviation over the realisations
+  std::vector<double> kmean, mean, stdev;
 };
 
 void ensure_static(Engine& E) {
@@ -893,13 +899,36 @@ Result run(Engine& E, const Request& q) {
   }
   HIPCHECK(hipEventRecord(E.dev.ev[2], st));
   HIPCHECK(bgw::launch_reduce(ra, static_cast<int>(out.n_groups), st));
+  const size_t n_real = E.real.size();
+  if (q.ensemble_stats) {
+    E.d_kmean.ensure(std::max<size_t>(1, n_real * nq) * sizeof(double));
+    E.d_emean.ensure(std::max<size_t>(1, nq) * sizeof(double));
+    E.d_estd.ensure(std::max<size_t>(1, nq) * sizeof(double));
+    HIPCHECK(bgw::launch_ensemble_stats(E.d_s.as<double>(), static_cast<int>(n_real),
+                                        static_cast<int>(q.k.size()), static_cast<int>(nq),
+                                        E.d_kmean.as<double>(), E.d_emean.as<double>(),
+                                        E.d_estd.as<double>(), st));
+  }
   HIPCHECK(hipEventRecord(E.dev.ev[3], st));
-  out.c.resize(out.n_groups * 2 * nq * 2);
   out.s.resize(out.n_groups * nq);
-  HIPCHECK(hipMemcpyAsync(out.c.data(), E.d_c.p, out.c.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
+  if (q.want_c) {
+    out.c.resize(out.n_groups * 2 * nq * 2);
+    HIPCHECK(hipMemcpyAsync(out.c.data(), E.d_c.p, out.c.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  }
   HIPCHECK(hipMemcpyAsync(out.s.data(), E.d_s.p, out.s.size() * sizeof(double),
                           hipMemcpyDeviceToHost, st));
+  if (q.ensemble_stats) {
+    out.kmean.resize(n_real * nq);
+    out.mean.resize(nq);
+    out.stdev.resize(nq);
+    HIPCHECK(hipMemcpyAsync(out.kmean.data(), E.d_kmean.p, out.kmean.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(out.mean.data(), E.d_emean.p, nq * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(out.stdev.data(), E.d_estd.p, nq * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  }
   HIPCHECK(hipStreamSynchronize(st));
   E.timings.geometry_ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
   E.timings.spectrum_ms = elapsed_ms(E.dev.ev[1], E.dev.ev[2]);
@@ -1475,6 +1504,7 @@ struct bgw_ensemble {
   Scaling sc;
   Engine engine;
   BgwTileConfig tiles{0, 0, 0, 0};
+  BgwTileConfig matrix_tiles{0, 0, 0, 0};  // the FP64 matrix form's (its own autotune)
   bool tuned = false;
 };
 
@@ -1723,9 +1753,6 @@ Request ensemble_request(bgw_ensemble* ensemble, const double* w_arr, size_t n_w
   slice_from_raw(t_begin, n_real, "t_begin");
   slice_from_raw(t_end, n_real, "t_end");
   if (n_t < 2) throw CalcError("Invalid resolution: n_t must be >= 2 for integration");
-  if (precision == BgwPrecision_Fp64Matrix)
-    throw CalcError("the FP64 matrix form runs single configurations only (not the batched "
-                    "ensembles): use fp64_blocked or mixed");
   check_unit_vectors(k_hat, n_k);
   Request q;
   q.res = &need_resolution(ensemble->res);
@@ -2323,7 +2350,19 @@ BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_
     check_tiles(t);
     Request q = ensemble_request(ensemble, w_arr, n_w, k_hat, n_k, t_begin, t_end, n_t, precision);
     EnsResult er;
-    if (n_k > 0 && n_real > 0 && nq > 0) {
+    if (n_k > 0 && n_real > 0 && nq > 0 && q.mode == 3) {
+      // the FP64 matrix form: the single-configuration launches over every
+      // realisation at once (tiles per realisation and bubble, so each
+      // realisation's spectrum is the one of its own configuration)
+      apply_tiles(tiles ? *tiles : ensemble->matrix_tiles, q);
+      q.want_c = false;
+      q.ensemble_stats = true;
+      Result r = run(ensemble->engine, q);
+      er.s = std::move(r.s);
+      er.kmean = std::move(r.kmean);
+      er.mean = std::move(r.mean);
+      er.stdev = std::move(r.stdev);
+    } else if (n_k > 0 && n_real > 0 && nq > 0) {
       er = run_ensemble(ensemble->engine, q, t, nullptr, true, 1);
     } else {
       er.s.assign(n_real * n_k * nq, 0.0);
@@ -2352,6 +2391,31 @@ BgwStatus bgw_ensemble_autotune(bgw_ensemble* ensemble, const double* w_arr, siz
     const size_t n_groups = ensemble->reals.size() * n_k;
     if (n_groups == 0 || n_w == 0 || ensemble->sc.n_sets == 0) {
       if (best) *best = ensemble->tiles;
+      return;
+    }
+    if (q.mode == 3) {
+      // The FP64 matrix form: 16-omega tiles per wavefront of the matrix kernel
+      // (1, 2 or 4), each timed on every m-th realisation (at most 32).
+      const size_t n_real = ensemble->reals.size();
+      const size_t m = std::max<size_t>(1, (n_real + 31) / 32);
+      Request qs = q;
+      for (size_t r = 0; r < n_real; ++r)
+        if (r % m != 0) qs.bubbles_of_real[r].clear();
+      qs.want_c = false;
+      BgwTileConfig bt{0, 0, 0, 0};
+      double best_ms = std::numeric_limits<double>::infinity();
+      for (size_t ot : {1, 2, 4}) {
+        qs.tile_rb = static_cast<int>(ot);
+        (void)run(ensemble->engine, qs);
+        (void)run(ensemble->engine, qs);
+        const double ms = ensemble->engine.timings.total_ms;
+        if (ms < best_ms) {
+          best_ms = ms;
+          bt.block_steps = ot;
+        }
+      }
+      ensemble->matrix_tiles = bt;
+      if (best) *best = bt;
       return;
     }
     // Candidates: workgroup size (omegas per lane 1, 2 or 4) x block length;

@@ -185,9 +185,55 @@ def test_setters_and_new_directions_between_calls(ens, chunk):
         assert got.tobytes() == want.tobytes(), (n_c, n_phi, coef, damping)
 
 
-def test_matrix_form_is_refused_for_ensembles(ens):
+@pytest.mark.parametrize("tiles", [None, {"block_steps": 1}, {"block_steps": 4}])
+def test_matrix_form_batched_equals_per_realisation(ens, tiles):
+    """The FP64 matrix form over the whole ensemble (the single-configuration launches
+    over every realisation at once): each realisation's spectrum bit-equal to its own
+    configuration's, the k-hat means and the ensemble mean and spread as the other forms
+    compute them, and S within 1e-10 of the oracle."""
+    inter, ext, t_end, w, k = ens
+    coef, pows = [[0.9]], [[5.2]]
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params(coef, pows)
+    out = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix", tiles=tiles)
+    for r in range(len(inter)):
+        bf = single(inter[r], ext[r], coef, pows)
+        if tiles:
+            bf.set_tiles(tiles["block_steps"], 0)
+        S, _ = bf.compute_spectrum(w[r], k, t_end=t_end[r], n_t=N_T, precision="fp64_matrix")
+        assert out["S"][r].tobytes() == S.tobytes(), r
+    S = out["S"]
+    km = np.zeros(S.shape[:1] + S.shape[2:])
+    for j in range(S.shape[1]):
+        km += S[:, j]
+    km /= S.shape[1]
+    np.testing.assert_array_equal(out["kmean"], km)
+    m = np.zeros(km.shape[1:])
+    for r in range(len(km)):
+        m += km[r]
+    m /= len(km)
+    v = np.zeros_like(m)
+    for r in range(len(km)):
+        v += (km[r] - m) ** 2
+    np.testing.assert_array_equal(out["mean"], m)
+    np.testing.assert_array_equal(out["std"], np.sqrt(v / len(km)))
+    for r in (0, 4):
+        So = oracle.spectrum(inter[r], ext[r], k, N_C, N_PHI, coef, pows, w[r],
+                             t_end=t_end[r], n_t=N_T)
+        assert np.max(np.abs(S[r] - So) / np.abs(So)) <= 1e-10, r
+
+
+def test_matrix_form_autotune(ens):
     inter, ext, t_end, w, k = ens
     E = BubbleEnsemble(inter, ext)
     E.set_resolution(N_C, N_PHI)
-    with pytest.raises(ValueError, match="single configurations only"):
-        E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    best = E.autotune(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    assert best["block_steps"] in (1, 2, 4)
+    a = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    b = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix",
+                           tiles={"block_steps": best["block_steps"]})
+    assert a["S"].tobytes() == b["S"].tobytes()
+    # the other forms keep their own launch shape
+    c = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    assert np.all(np.isfinite(c["S"]))

@@ -10,8 +10,9 @@
   the highest omegas.
 * Four realisations of the 512-member ensemble at its grids (32 x 64, 500
   steps, 128 omegas / L), the largest (395 bubbles) among them, through the
-  batched path in its fused and pipelined forms: each realisation bit-equal to
-  the single-configuration path and within 1e-10 per bin of the oracle.
+  batched path in its fused and pipelined forms and in the FP64 matrix form: each
+  realisation bit-equal to the single-configuration path and within 1e-10 per bin
+  of the oracle.
 
 The fixtures come from tests/golden/make_golden.py.
 """
@@ -114,6 +115,8 @@ def members():
     # pipelined chunks of about 512 items: several chunks, geometry and phase sums
     # on two streams
     ("fp64_blocked", {"block_steps": 8, "threads": 64, "items_per_chunk": 512}),
+    # the FP64 matrix form: the single-configuration launches over all four at once
+    ("fp64_matrix", {"block_steps": 2, "threads": 0}),
 ])
 def test_ensemble_members_full_size(members, precision, tiles):
     g = members
