@@ -276,7 +276,7 @@ def ensemble_bench(ens, steps):
     pts = n_real * len(ens.k_hat) * w.shape[1]
     out, res = {}, {}
     tuned = {}
-    for m in ("mixed", "fp64_blocked", "fp64"):
+    for m in ("mixed", "fp64_blocked", "fp64_matrix", "fp64"):
         if m == "fp64":
             tl = dict(tuned["fp64_blocked"])
             tl["block_steps"] = 16
@@ -287,30 +287,41 @@ def ensemble_bench(ens, steps):
             tuned[m] = {k: v for k, v in tl.items() if k != "autotune_s"}
         tiles = {k: v for k, v in tl.items() if k != "autotune_s"}
         r = E.compute_spectrum(w, ens.k_hat, t_end=ens.t_end, n_t=ens.n_t, precision=m, tiles=tiles)
-        ms = []
+        ms, wall = [], []
         for _ in range(max(1, steps)):
+            t1 = time.perf_counter()
             r = E.compute_spectrum(w, ens.k_hat, t_end=ens.t_end, n_t=ens.n_t, precision=m,
                                    tiles=tiles)
+            wall.append(time.perf_counter() - t1)
             ms.append(E.last_timings()["total_ms"])
         res[m] = r
         out[m] = {"device_ms": float(np.mean(ms)), "points_per_s": pts / (np.mean(ms) * 1e-3),
-                  "tiles": tl}
+                  "wall_ms": float(np.mean(wall)) * 1e3, "tiles": tl}
         print(f"# ensemble {m}: {out[m]}", file=sys.stderr, flush=True)
     ref = res["fp64"]
-    for m in ("mixed", "fp64_blocked"):
+    for m in ("mixed", "fp64_blocked", "fp64_matrix"):
         rel = np.abs(res[m]["S"] - ref["S"]) / np.abs(ref["S"])
         out[m]["err_vs_fp64"] = {
             "max": float(rel.max()), "median": float(np.median(rel)),
             "frac_le_1e-6": float(np.mean(rel <= REL_BAR)),
             "ensemble_mean_max": float(np.max(np.abs(res[m]["mean"] - ref["mean"]) / np.abs(ref["mean"])))}
         out[m]["speedup_vs_fp64"] = out[m]["points_per_s"] / out["fp64"]["points_per_s"]
+    # the batched target: the fastest form whose largest per-bin error is within the bar
+    ok = [m for m in ("mixed", "fp64_blocked", "fp64_matrix") if out[m]["err_vs_fp64"]["max"] <= REL_BAR]
+    best = max(ok, key=lambda m: out[m]["points_per_s"]) if ok else None
     counts = np.array([len(b) for b in ens.interiors])
     del E  # release the batch's device buffers before the rest of the run
     return {"workload": "ensemble_512: 512 realisations, beta L ~ U[10,30], bubble counts "
                         f"{int(counts.min())}-{int(counts.max())} (mean {counts.mean():.1f}), "
                         "32x64, 500 t to each percolation time, 128 omegas geomspace(0.5,300)/L, "
-                        "32 k-hat; one graph-captured launch sequence per call",
+                        "32 k-hat; mixed / blocked / FP64: one graph-captured launch sequence "
+                        "per call; FP64 matrix form: the single-configuration launches over "
+                        "every realisation at once",
             "points": pts, "setup_s": build_s, "modes": out,
+            "best_within_1e-6": None if best is None else {
+                "form": best, "points_per_s": out[best]["points_per_s"],
+                "speedup_vs_fp64": out[best]["speedup_vs_fp64"],
+                "max_rel_err_vs_fp64": out[best]["err_vs_fp64"]["max"]},
             "ensemble_mean_spread": {"mean_S_first_bin": float(ref["mean"][0, 0]),
                                      "std_S_first_bin": float(ref["std"][0, 0])}}
 

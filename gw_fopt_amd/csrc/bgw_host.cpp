@@ -616,11 +616,13 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
     }
   }
   const size_t n_items = items.size();
-  // launch order of the phase-sum kernel: most expensive items first
+  // launch order of the per-item phase-sum kernels: most expensive items first
+  // (the matrix form orders its tiles itself)
   std::vector<int32_t> order(n_items);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+  if (q.mode != 3)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
 
   const size_t n_dir = res.n_c * res.n_phi;
   const size_t nq = n_sets * q.n_w;
