@@ -376,8 +376,11 @@ BgwStatus bgw_ensemble_set_gradient_scaling_params(bgw_ensemble *ensemble,
  * kmean_out (optional): (n_real, n_sets, n_w) mean over k-hat;
  * ens_mean_out / ens_std_out (optional): (n_sets, n_w) mean and population
  * standard deviation over realisations of kmean.  The whole batch runs as
- * one captured hipGraph (persistent kernels over a work queue); tiles NULL
- * uses the autotuned (or default) launch shape. */
+ * one captured hipGraph (persistent kernels over a work queue, or pipelined
+ * chunks); BgwPrecision_Fp64Matrix runs the single-configuration matrix-form
+ * launches over every realisation at once instead (not captured; each
+ * realisation's spectrum equals its own configuration's).  tiles NULL uses the
+ * autotuned (or default) launch shape of the form. */
 BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble *ensemble, const double *w_arr, size_t n_w,
                                         const double *k_hat, size_t n_k, const double *t_begin,
                                         const double *t_end, size_t n_t,
