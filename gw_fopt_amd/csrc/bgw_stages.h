@@ -362,16 +362,18 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       // hdsq > lim (1 + 1e-12) dot proves hdsq / dot > lim (the relative slack
       // exceeds the rounding of both sides), so the division runs only when a
       // candidate can lower the best or the second delay.
-      double best[D], second[D], stop_lim[D], div_lim[D];
-      int bidx[D], bpos[D];
+      double best[D], second[D], stop_lim[D], div_lim[D], bscale[D];
+      int bidx[D], bpos[D], bflags[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         best[d] = kInf;
         second[d] = kInf;
         stop_lim[d] = kInf;
         div_lim[d] = kInf;
+        bscale[d] = 0.0;
         bidx[d] = 0x7fffffff;
         bpos[d] = -1;
+        bflags[d] = 0;
       }
       // irregular candidates (key -inf) sort first
       const bool irregular = n_cand > 0 && !(cand_lds[0].flags & kCandRegular);
@@ -402,6 +404,9 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
                 best[d] = dtv;
                 bidx[d] = e.idx;
                 bpos[d] = s;
+                // the minimum's scale and flags, for the verification rule below
+                bscale[d] = fabs(e.d0) + sqrt(e.d1 * e.d1 + e.d2 * e.d2 + e.d3 * e.d3);
+                bflags[d] = e.flags;
               } else {
                 second[d] = fmin(second[d], dtv);
               }
@@ -432,20 +437,15 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       // nothing needs testing at all.  The wave walks a prefix of the list
       // only when one of its directions needs it.
       bool p2[D], ok[D];
-      RotCand b0[D];
       double thr[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         const bool minimum = act[d] && !fb[d] && bpos[d] >= 0;
         ok[d] = true;
         thr[d] = kInf;
-        b0[d] = RotCand{};
         if (minimum) {
-          b0[d] = fetch_cand(bpos[d], cand_lds, n_lds, list, R, identity);
-          const double scale =
-              fabs(b0[d].d0) + sqrt(b0[d].d1 * b0[d].d1 + b0[d].d2 * b0[d].d2 + b0[d].d3 * b0[d].d3);
-          const bool regular = (b0[d].flags & kCandRegular) != 0;
-          const bool grazing = !(best[d] <= 1e3 * scale);
+          const bool regular = (bflags[d] & kCandRegular) != 0;
+          const bool grazing = !(best[d] <= 1e3 * bscale[d]);
           if (regular && !grazing) thr[d] = best[d] * (1.0 + 1e-4);
         }
         p2[d] = minimum && (irregular || !(second[d] > thr[d]));
@@ -453,21 +453,25 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       bool any_p2 = false;
 #pragma unroll
       for (int d = 0; d < D; ++d) any_p2 = any_p2 || p2[d];
-      auto step2 = [&](const RotCand& e, int s) -> bool {
-        bool need[D], any_need = false;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          need[d] = p2[d] && ok[d] && !(e.key > thr[d]);
-          any_need = any_need || need[d];
-        }
-        if (!wave_any(any_need)) return false;
+      if (wave_any(any_p2)) {
+        RotCand b0[D];
 #pragma unroll
         for (int d = 0; d < D; ++d)
-          if (need[d] && s != bpos[d] && !collision_point_ok(b0[d], e, x1[d], x2[d], x3[d]))
-            ok[d] = false;
-        return true;
-      };
-      if (wave_any(any_p2)) {
+          b0[d] = p2[d] ? fetch_cand(bpos[d], cand_lds, n_lds, list, R, identity) : RotCand{};
+        auto step2 = [&](const RotCand& e, int s) -> bool {
+          bool need[D], any_need = false;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            need[d] = p2[d] && ok[d] && !(e.key > thr[d]);
+            any_need = any_need || need[d];
+          }
+          if (!wave_any(any_need)) return false;
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+            if (need[d] && s != bpos[d] && !collision_point_ok(b0[d], e, x1[d], x2[d], x3[d]))
+              ok[d] = false;
+          return true;
+        };
         int s = 0;
         bool more = true;
         for (; more && s < n_lds; ++s) {
@@ -491,11 +495,15 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
         if (active[d]) {
           double dtab = 0.0;  // compute_delta_tab (:446-468)
           if (result >= 0) {
-            // the verified minimum was fetched already; fallback lanes look theirs up
-            const RotCand eb =
-                fb[d] ? fetch_cand(inv[result], cand_lds, n_lds, list, R, identity) : b0[d];
-            const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1[d], x2[d], x3[d]);
-            dtab = fabs(dot) < 1e-10 ? 0.0 : eb.hdsq / dot;
+            if (fb[d]) {
+              const RotCand eb = fetch_cand(inv[result], cand_lds, n_lds, list, R, identity);
+              const double dot = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1[d], x2[d], x3[d]);
+              dtab = fabs(dot) < 1e-10 ? 0.0 : eb.hdsq / dot;
+            } else {
+              // the verified minimum's delay: pass 1 computed it with the same operands
+              // and operations (its dot > 1e-10)
+              dtab = best[d];
+            }
             if (dtab > 0.0) tau = dtab;  // AlreadyCollided needs delta_tab > 0 (:405)
           }
           const int64_t dir = static_cast<int64_t>(ring[d]) * n_phi + jd[d];
