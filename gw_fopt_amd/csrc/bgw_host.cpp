@@ -738,8 +738,16 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
 // matrix kernel from q.tile_rb (1, 2 or 4; 0: 2).
 constexpr size_t kMatrixRowBytes = size_t(16) << 30;
 
-void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw::SpecArgs& sb,
-                        hipStream_t st) {
+// The launches of the matrix form for one set group: the arguments of every
+// batch of tiles (the planned buffers already uploaded).
+struct MatrixPlan {
+  bgw::MatArgs ma{};
+  std::vector<std::pair<size_t, size_t>> batches;  // tile ranges
+  std::vector<int64_t> voff0;                      // weight-row offset of each batch
+};
+
+MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
+                            const bgw::SpecArgs& sb, hipStream_t st) {
   if (sb.has_damping)
     throw CalcError("the FP64 matrix form does not take a damping width (use fp64_blocked)");
   if (q.n_t > 65535) throw CalcError("the FP64 matrix form supports n_t <= 65535");
@@ -826,21 +834,38 @@ void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw
   }
   E.d_vbuf.ensure((max_batch + static_cast<size_t>(bgw::kMatrixRowPad) * 32) * sizeof(double));
   ma.vbuf = E.d_vbuf.as<double>();
-  for (const auto& b : batches) {
-    ma.tile0 = static_cast<int>(b.first);
-    ma.voff0 = info[b.first].voff;
-    const int nt = static_cast<int>(b.second - b.first);
+  MatrixPlan P;
+  P.ma = ma;
+  P.batches = batches;
+  for (const auto& b : batches) P.voff0.push_back(info[b.first].voff);
+  return P;
+}
+
+// The plan's launches on st: per batch the time tables and weight rows, then
+// the products (timed with events unless `timed` is false, as under capture).
+void launch_matrix_plan(Engine& E, const MatrixPlan& P, hipStream_t st, bool timed) {
+  bgw::MatArgs ma = P.ma;
+  for (size_t b = 0; b < P.batches.size(); ++b) {
+    ma.tile0 = static_cast<int>(P.batches[b].first);
+    ma.voff0 = P.voff0[b];
+    const int nt = static_cast<int>(P.batches[b].second - P.batches[b].first);
     HIPCHECK(bgw::launch_matrix_weights(ma, nt, st));
     const int slot = E.mx_timed;
-    if (slot < Device::kTimedProducts) HIPCHECK(hipEventRecord(E.dev.mx_ev[2 * slot], st));
+    const bool ev = timed && slot < Device::kTimedProducts;
+    if (ev) HIPCHECK(hipEventRecord(E.dev.mx_ev[2 * slot], st));
     HIPCHECK(bgw::launch_matrix_products(ma, nt, st));
-    if (slot < Device::kTimedProducts) {
+    if (ev) {
       HIPCHECK(hipEventRecord(E.dev.mx_ev[2 * slot + 1], st));
       ++E.mx_timed;
     } else {
       E.mx_untimed = true;
     }
   }
+}
+
+void launch_matrix_form(Engine& E, const Request& q, const Launch& Lc, const bgw::SpecArgs& sb,
+                        hipStream_t st) {
+  launch_matrix_plan(E, plan_matrix_form(E, q, Lc, sb, st), st, true);
 }
 
 Result run(Engine& E, const Request& q) {
@@ -1127,6 +1152,14 @@ void key_args(std::ostringstream& k, const bgw::RedArgs& a) {
   k << "|R" << a.partial << ',' << a.group_off << ',' << a.group_r << ',' << a.real << ','
     << a.omega << ',' << a.n_sets << ',' << a.n_w << ',' << a.c_out << ',' << a.s_out << ','
     << a.unit_group << ',' << a.unit_off << ',' << a.u0 << ',' << a.item0;
+}
+
+void key_args(std::ostringstream& k, const bgw::MatArgs& a) {
+  static_assert(sizeof(bgw::MatArgs) == 296, "MatArgs changed: list the new field in key_args");
+  key_args(k, a.s);
+  k << "|M" << a.tiles << ',' << a.tinfo << ',' << a.ni << ',' << a.omega_tiles << ','
+    << a.col_tiles << ',' << a.tile0 << ',' << a.voff0 << ',' << a.vbuf << ',' << a.tt << ','
+    << a.ctlo;
 }
 
 void key_args(std::ostringstream& k, const bgw::EnsArgs& a) {
@@ -1426,6 +1459,110 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   E.timings.n_items = Lc.n_items;
   E.timings.n_directions = Lc.n_items * n_dir;
   E.timings.n_terms = Lc.n_items * res.n_c * q.n_t * q.n_w;
+  return out;
+}
+
+// A batched ensemble in the FP64 matrix form: the single-configuration launches
+// over every realisation at once (geometry, per batch of tiles the time tables,
+// weight rows and products, the reduction, then the k-hat means and the
+// ensemble mean and spread), captured as one hipGraph after the per-call
+// uploads and replayed while every argument of its launches is unchanged
+// (key_args).  Requests whose sets need several matrix launches run uncaptured.
+Result run_matrix_ensemble(Engine& E, const Request& q) {
+  const Scaling& sc = *q.sc;
+  const size_t n_sets = sc.n_sets;
+  if (n_sets == 0 || q.n_w == 0 || n_sets > kMaxSetsPerLaunch ||
+      n_sets * sc.n_terms > static_cast<size_t>(bgw::kMaxMatrixTerms))
+    return run(E, q);
+  Launch Lc;
+  setup(E, q, Lc);
+  hipStream_t st = E.dev.stream;
+  const size_t n_real = E.real.size(), n_k = q.k.size();
+  const size_t nq = n_sets * q.n_w;
+  bgw::SpecArgs sm = Lc.sa;
+  sm.set0 = 0;
+  sm.n_sets = static_cast<int>(n_sets);
+  sm.coef = E.d_coef.as<double>();
+  sm.pows = E.d_pows.as<double>();
+  sm.time_tables = 0;
+  const MatrixPlan P = plan_matrix_form(E, q, Lc, sm, st);
+  E.d_kmean.ensure(std::max<size_t>(1, n_real * nq) * sizeof(double));
+  E.d_emean.ensure(std::max<size_t>(1, nq) * sizeof(double));
+  E.d_estd.ensure(std::max<size_t>(1, nq) * sizeof(double));
+  std::ostringstream key;
+  key << "matrix:" << Lc.n_items << ':' << Lc.n_groups << ':' << n_real << ':' << n_k << ':' << nq;
+  key_args(key, Lc.ga);
+  for (size_t b = 0; b < P.batches.size(); ++b) {
+    bgw::MatArgs ma = P.ma;
+    ma.tile0 = static_cast<int>(P.batches[b].first);
+    ma.voff0 = P.voff0[b];
+    key << "|b" << (P.batches[b].second - P.batches[b].first);
+    key_args(key, ma);
+  }
+  key_args(key, Lc.ra);
+  key << ':' << static_cast<const void*>(E.d_kmean.p) << ':' << static_cast<const void*>(E.d_emean.p)
+      << ':' << static_cast<const void*>(E.d_estd.p);
+  if (!E.graph || E.graph_key != key.str()) {
+    if (E.graph) {
+      HIPCHECK(hipGraphExecDestroy(E.graph));
+      E.graph = nullptr;
+    }
+    E.graph_key.clear();
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t err = bgw::launch_geometry(Lc.ga, static_cast<int>(Lc.n_items), st);
+    if (err == hipSuccess) {
+      try {
+        launch_matrix_plan(E, P, st, false);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+    }
+    if (err == hipSuccess) err = bgw::launch_reduce(Lc.ra, static_cast<int>(Lc.n_groups), st);
+    if (err == hipSuccess)
+      err = bgw::launch_ensemble_stats(E.d_s.as<double>(), static_cast<int>(n_real),
+                                       static_cast<int>(n_k), static_cast<int>(nq),
+                                       E.d_kmean.as<double>(), E.d_emean.as<double>(),
+                                       E.d_estd.as<double>(), st);
+    hipError_t end = hipStreamEndCapture(st, &graph);
+    HIPCHECK(err);
+    HIPCHECK(end);
+    hipError_t inst = hipGraphInstantiate(&E.graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(inst);
+    E.graph_key = key.str();
+  }
+  HIPCHECK(hipEventRecord(E.dev.ev[0], st));
+  HIPCHECK(hipGraphLaunch(E.graph, st));
+  HIPCHECK(hipEventRecord(E.dev.ev[1], st));
+  Result out;
+  out.n_groups = Lc.n_groups;
+  out.s.resize(Lc.n_groups * nq);
+  out.kmean.resize(n_real * nq);
+  out.mean.resize(nq);
+  out.stdev.resize(nq);
+  HIPCHECK(hipMemcpyAsync(out.s.data(), E.d_s.p, out.s.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(out.kmean.data(), E.d_kmean.p, out.kmean.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(out.mean.data(), E.d_emean.p, nq * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(out.stdev.data(), E.d_estd.p, nq * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  const double ms = elapsed_ms(E.dev.ev[0], E.dev.ev[1]);
+  const size_t n_dir = q.res->n_c * q.res->n_phi;
+  E.timings.geometry_ms = 0.0;
+  E.timings.spectrum_ms = ms;
+  E.timings.reduce_ms = 0.0;
+  E.timings.matrix_ms = std::nan("");
+  E.timings.total_ms = ms;
+  E.timings.n_items = Lc.n_items;
+  E.timings.n_directions = Lc.n_items * n_dir;
+  E.timings.n_terms = Lc.n_items * q.res->n_c * q.n_t * q.n_w;
   return out;
 }
 
@@ -2359,7 +2496,7 @@ BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble* ensemble, const double* w_
       apply_tiles(tiles ? *tiles : ensemble->matrix_tiles, q);
       q.want_c = false;
       q.ensemble_stats = true;
-      Result r = run(ensemble->engine, q);
+      Result r = run_matrix_ensemble(ensemble->engine, q);
       er.s = std::move(r.s);
       er.kmean = std::move(r.kmean);
       er.mean = std::move(r.mean);

@@ -378,8 +378,9 @@ BgwStatus bgw_ensemble_set_gradient_scaling_params(bgw_ensemble *ensemble,
  * standard deviation over realisations of kmean.  The whole batch runs as
  * one captured hipGraph (persistent kernels over a work queue, or pipelined
  * chunks); BgwPrecision_Fp64Matrix runs the single-configuration matrix-form
- * launches over every realisation at once instead (not captured; each
- * realisation's spectrum equals its own configuration's).  tiles NULL uses the
+ * launches over every realisation at once instead (captured as one hipGraph
+ * too, unless its sets need several matrix launches; each realisation's
+ * spectrum equals its own configuration's).  tiles NULL uses the
  * autotuned (or default) launch shape of the form. */
 BgwStatus bgw_ensemble_compute_spectrum(bgw_ensemble *ensemble, const double *w_arr, size_t n_w,
                                         const double *k_hat, size_t n_k, const double *t_begin,

@@ -185,14 +185,20 @@ def test_setters_and_new_directions_between_calls(ens, chunk):
         assert got.tobytes() == want.tobytes(), (n_c, n_phi, coef, damping)
 
 
-@pytest.mark.parametrize("tiles", [None, {"block_steps": 1}, {"block_steps": 4}])
-def test_matrix_form_batched_equals_per_realisation(ens, tiles):
+@pytest.mark.parametrize("tiles,coef,pows", [
+    (None, [[0.9]], [[5.2]]),
+    ({"block_steps": 1}, [[0.9]], [[5.2]]),
+    ({"block_steps": 4}, [[0.9]], [[5.2]]),
+    # two sets of two terms (one tile column per (item, set, +/x)), integer and
+    # non-integer powers
+    (None, [[0.5, 0.4], [1.0, 0.2]], [[5.0, 4.0], [5.0, 6.5]]),
+])
+def test_matrix_form_batched_equals_per_realisation(ens, tiles, coef, pows):
     """The FP64 matrix form over the whole ensemble (the single-configuration launches
     over every realisation at once): each realisation's spectrum bit-equal to its own
     configuration's, the k-hat means and the ensemble mean and spread as the other forms
     compute them, and S within 1e-10 of the oracle."""
     inter, ext, t_end, w, k = ens
-    coef, pows = [[0.9]], [[5.2]]
     E = BubbleEnsemble(inter, ext)
     E.set_resolution(N_C, N_PHI)
     E.set_gradient_scaling_params(coef, pows)
@@ -237,3 +243,55 @@ def test_matrix_form_autotune(ens):
     # the other forms keep their own launch shape
     c = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
     assert np.all(np.isfinite(c["S"]))
+
+def test_matrix_form_graph_replays(ens):
+    """The batched matrix form is captured once and replayed while its launches are
+    unchanged: replays with other GPU work in between are bit-equal, and replays after
+    new scaling parameters or percolation times (same launch shapes, new contents of the
+    uploaded tables) equal a fresh ensemble's first, captured, call."""
+    inter, ext, t_end, w, k = ens
+    coef, pows = [[0.9]], [[5.2]]
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params(coef, pows)
+    a = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    bf = single(inter[4], ext[4], coef, pows)
+    bf.compute_spectrum(w[4], k, t_end=t_end[4], n_t=N_T, precision="fp64_matrix")
+    b = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="mixed")
+    bf.compute_c_integral(w[4], t_end=t_end[4], n_t=N_T)
+    c = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    for key in ("S", "kmean", "mean", "std"):
+        assert a[key].tobytes() == b[key].tobytes() == c[key].tobytes(), key
+
+    coef2, pows2 = [[1.3]], [[4.6]]
+    t2 = t_end * 1.05
+    E.set_gradient_scaling_params(coef2, pows2)
+    d = E.compute_spectrum(w, k, t_end=t2, n_t=N_T, precision="fp64_matrix")
+    F = BubbleEnsemble(inter, ext)
+    F.set_resolution(N_C, N_PHI)
+    F.set_gradient_scaling_params(coef2, pows2)
+    f = F.compute_spectrum(w, k, t_end=t2, n_t=N_T, precision="fp64_matrix")
+    for key in ("S", "kmean", "mean", "std"):
+        assert d[key].tobytes() == f[key].tobytes(), key
+    assert d["S"].tobytes() != a["S"].tobytes()
+
+
+def test_matrix_form_many_terms_uncaptured(ens):
+    """Two sets of seven terms take two matrix launches per call (at most twelve
+    (set, term) pairs each): that request runs uncaptured and still equals each
+    realisation's own configuration bit for bit."""
+    inter, ext, t_end, w, k = ens
+    pw = [5.0, 4.5, 4.0, 3.5, 3.0, 2.5, 2.0]
+    coef = [[0.3, 0.1, 0.2, 0.05, 0.1, 0.02, 0.01], [0.2, 0.2, 0.1, 0.1, 0.05, 0.05, 0.02]]
+    pows = [pw, pw[::-1]]
+    E = BubbleEnsemble(inter, ext)
+    E.set_resolution(N_C, N_PHI)
+    E.set_gradient_scaling_params(coef, pows)
+    out = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    again = E.compute_spectrum(w, k, t_end=t_end, n_t=N_T, precision="fp64_matrix")
+    assert out["S"].tobytes() == again["S"].tobytes()
+    for r in (0, 2, 7):
+        bf = single(inter[r], ext[r], coef, pows)
+        S, _ = bf.compute_spectrum(w[r], k, t_end=t_end[r], n_t=N_T, precision="fp64_matrix")
+        assert out["S"][r].tobytes() == S.tobytes(), r
