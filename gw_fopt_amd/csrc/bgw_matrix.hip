@@ -199,10 +199,11 @@ __global__ __launch_bounds__(256) void mx_time_kernel(MatArgs m) {
 //    [g 2^SLS, (g + 1) 2^SLS); each sums its segment, a 32-lane scan of the
 //    segment sums gives every segment's start, and a second walk writes the
 //    records.
-//  * rows: lane t writes column 16 ct + t % 16 of rows from the column tile's
-//    first weighted step (rounded down to 4; the products read nothing of the
-//    column tile before it), 16 rows apart; the collided count is found once
-//    by a binary search and carried along the lane's rows.
+//  * rows: lane t writes the two columns (+, x) 16 ct + 2 (t % 8) and the
+//    next of rows from the column tile's first weighted step (rounded down to
+//    4; the products read nothing of the column tile before it), 32 rows
+//    apart; the collided count is found once by a binary search and carried
+//    along the lane's rows.
 // FAST: one set of one term.
 template <bool FAST>
 __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
@@ -362,13 +363,16 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
     lo_s = lo;
   }
   __syncthreads();
-  // (4) this column tile's weight rows from the first one the products read
-  const int col = 16 * ct + (tid & 15);
-  const int jg = col / cpi;           // the column's item in the tile
+  // (4) this column tile's weight rows from the first one the products read:
+  // lane t writes the two columns (+, x) of one (item, set), 16 ct + 2 (t % 8)
+  // and the next, of rows 32 apart, sharing the collided count, the record and
+  // the time-table loads of a row between them
+  const int col = 16 * ct + 2 * (tid & 7);
+  const int jg = col / cpi;           // the columns' item in the tile
   const int jc = jg - j0;             // and in this workgroup
   const int live0 = lo_s < n_t ? (lo_s & ~3) - info.kbase : info.len;
-  const int RP = (info.len - live0 + 15) / 16;
-  const int r0 = live0 + (tid >> 4) * RP;
+  const int RP = (info.len - live0 + 31) / 32;
+  const int r0 = live0 + (tid >> 3) * RP;
   const int r1 = min(info.len, r0 + RP);
   double* vout = m.vbuf + (info.voff - m.voff0) + static_cast<int64_t>(i) * info.len * ncol;
   const bool live = jg < ni && jc < nj && item_s[jc] >= 0;
@@ -376,7 +380,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const double rf = live ? rf_s[jc] : 0.0;
   const double* Tc = tab + static_cast<size_t>(live ? jc : 0) * L.TS;
   const uint16_t* kr = kap + (live ? jc : 0) * L.KS;
-  const int sc = (col >> 1) % nsets, pm = col & 1;
+  const int sc = (col >> 1) % nsets;
   int cnt = -1;
   // groups of four rows: their time-table loads are issued together
   for (int rg = r0; rg < r1; rg += 4) {
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
       const int r = rg + u;
       if (r >= r1) break;
       const int k = info.kbase + r;
-      double v = 0.0;
+      double2 v = make_double2(0.0, 0.0);
       if (live && k < n_t && k >= tlo) {
         if (cnt < 0) {
           cnt = count_steps_le(kr, n_phi, k);
@@ -402,19 +406,22 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
         }
         const double* rec = Tc + L.off(cnt);
         const double base = tb4[u] * rf;
-        double b = rec[pm];
+        double bp = rec[0], bx = rec[1];
         if (FAST) {
-          b = __builtin_fma(tf4[u], rec[2 + pm], b);
+          bp = __builtin_fma(tf4[u], rec[2], bp);
+          bx = __builtin_fma(tf4[u], rec[3], bx);
         } else {
           const double* tk = ttile + static_cast<int64_t>(k) * TW;
           for (int kt = 0; kt < n_terms; ++kt) {
             const int q = sc * n_terms + kt;
-            b = __builtin_fma(tk[1 + q], rec[2 + 2 * q + pm], b);
+            const double f = tk[1 + q];
+            bp = __builtin_fma(f, rec[2 + 2 * q], bp);
+            bx = __builtin_fma(f, rec[3 + 2 * q], bx);
           }
         }
-        v = base * b;
+        v = make_double2(base * bp, base * bx);
       }
-      vout[static_cast<int64_t>(r) * ncol + col] = v;
+      *reinterpret_cast<double2*>(vout + static_cast<int64_t>(r) * ncol + col) = v;
     }
   }
 }
@@ -426,7 +433,26 @@ template <int OT, int CT>
 __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
   constexpr int NCOL = 16 * CT;
   const SpecArgs& g = m.s;
-  const int tile = m.tile0 + static_cast<int>(blockIdx.x);
+  // Workgroups go to the 8 XCDs in turn (dispatch index b to XCD b % 8): the
+  // omega blocks of one tile take dispatch indices 8 apart, so they run on the
+  // same XCD at about the same time and the second reads the tile's weight
+  // rows from that XCD's L2.  Tiles keep their cost order; the last
+  // (n_tiles % 8) tiles take the remaining indices in order.
+  const int ny = static_cast<int>(gridDim.y);
+  const int ntl = static_cast<int>(gridDim.x);
+  const int b = static_cast<int>(blockIdx.x) + static_cast<int>(blockIdx.y) * ntl;
+  const int full = (ntl / 8) * 8 * ny;
+  int tl, by;
+  if (b < full) {
+    const int slot = b / 8;
+    tl = (slot / ny) * 8 + b % 8;
+    by = slot % ny;
+  } else {
+    const int r = b - full;
+    tl = (ntl / 8) * 8 + r / ny;
+    by = r % ny;
+  }
+  const int tile = m.tile0 + tl;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kk = lane >> 4;
   const int n_t = g.n_t;
@@ -436,7 +462,7 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
   const RealInfo ri = g.real[it0.r];
   const double t_a = it0.t_a;
   const MxTileInfo info = m.tinfo[tile];
-  const int wbase = static_cast<int>(blockIdx.y) * (blockDim.x >> 6) * 16 * OT + wave * 16 * OT;
+  const int wbase = by * (blockDim.x >> 6) * 16 * OT + wave * 16 * OT;
   double om[OT];
 #pragma unroll
   for (int o = 0; o < OT; ++o) {
