@@ -443,3 +443,23 @@ def test_reference_example_configuration_conf_y():
     S, _ = bf.compute_spectrum(w, k, t_end=2.0, n_t=150)
     So = oracle.spectrum(inter, ext, k, 24, 48, [[1.0]], [[5.0]], w, t_end=2.0, n_t=150)
     assert rel_per_bin(S, So) <= FP64_REL
+
+
+@pytest.mark.parametrize("n_k", [5, 12, 40])
+@pytest.mark.parametrize("sets", [([[1.0]], [[5.0]]), ([[0.7, 0.3], [1.0, 0.1]], [[5.0, 3.5], [4.0, 6.0]])])
+@pytest.mark.parametrize("block_steps", [1, 2, 4])
+def test_matrix_form_column_tiles(n_k, sets, block_steps):
+    """The matrix form with one and two column tiles per tile (5, 12 and 40 k-hat slots;
+    one set fills 8 items per column tile, two sets 4), several tiles per bubble with a
+    partly filled last one, and 1, 2 and 4 omega tiles per wavefront, per bin within 1e-10
+    of the plain FP64 form."""
+    coef, pows = sets
+    b, ext = small_async(31, n=7)
+    k = mb.fibonacci_directions(n_k)
+    w = np.geomspace(0.5, 150.0, 40)
+    bf = flow(b, ext, 12, 24, coef, pows)
+    t_end = float(b[:, 0].max()) + 0.6
+    S64, _ = bf.compute_spectrum(w, k, t_end=t_end, n_t=120, precision="fp64")
+    bf.set_tiles(block_steps, 0)
+    Sm, _ = bf.compute_spectrum(w, k, t_end=t_end, n_t=120, precision="fp64_matrix")
+    assert rel_per_bin(Sm, S64) <= FP64_REL
