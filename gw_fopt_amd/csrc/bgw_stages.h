@@ -377,42 +377,56 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       }
       // irregular candidates (key -inf) sort first
       const bool irregular = n_cand > 0 && !(cand_lds[0].flags & kCandRegular);
+      // Candidates go in pairs (two per loop step: one exit ballot and one
+      // candidate ballot per pair).  The second of a pair is tested with the
+      // limits from before the first's update, which are never tighter: it may
+      // be divided where a single step would have skipped it, but a candidate
+      // that passes only the looser test can lower neither the best delay nor
+      // the second (its delay exceeds both), so the results are the same.
       // Two copies of the walk: staged candidates from LDS, the rest from HBM
       // (one loop over both would read the fields through generic pointers).
-      auto step1 = [&](const RotCand& e, int s) -> bool {
-        bool need[D], cand[D], any_need = false, any_cand = false;
-        double dot[D];
+      auto update = [&](const RotCand& e, int s, double dot, int d) {
+        const double dtv = e.hdsq / dot;
+        if (dtv != dtv) {
+          fb[d] = true;
+        } else if (dot > 1e-10 && dtv > 0.0 &&
+                   (dtv < best[d] || (dtv == best[d] && e.idx < bidx[d]))) {
+          second[d] = fmin(second[d], best[d]);
+          best[d] = dtv;
+          bidx[d] = e.idx;
+          bpos[d] = s;
+          // the minimum's scale and flags, for the verification rule below
+          bscale[d] = fabs(e.d0) + sqrt(e.d1 * e.d1 + e.d2 * e.d2 + e.d3 * e.d3);
+          bflags[d] = e.flags;
+        } else {
+          second[d] = fmin(second[d], dtv);
+        }
+        stop_lim[d] = best[d] * ((1.0 + 1e-4) * (1.0 + 1e-6));
+        div_lim[d] = fmax(best[d], second[d]) * (1.0 + 1e-12);
+      };
+      auto step1 = [&](const RotCand& ea, const RotCand& eb, int s, bool has_b) -> bool {
+        bool need_a[D], cand_a[D], cand_b[D], any_need = false, any_cand = false;
+        double dot_a[D], dot_b[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          need[d] = act[d] && !(e.key > stop_lim[d]);
-          any_need = any_need || need[d];
-          dot[d] = lscalar(e.d0, e.d1, e.d2, e.d3, 1.0, x1[d], x2[d], x3[d]);
-          cand[d] = need[d] && dot[d] > 0.0 && !(e.hdsq > div_lim[d] * dot[d]);
-          any_cand = any_cand || cand[d];
+          need_a[d] = act[d] && !(ea.key > stop_lim[d]);
+          any_need = any_need || need_a[d];
         }
         if (!wave_any(any_need)) return false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const bool need_b = has_b && act[d] && !(eb.key > stop_lim[d]);
+          dot_a[d] = lscalar(ea.d0, ea.d1, ea.d2, ea.d3, 1.0, x1[d], x2[d], x3[d]);
+          dot_b[d] = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1[d], x2[d], x3[d]);
+          cand_a[d] = need_a[d] && dot_a[d] > 0.0 && !(ea.hdsq > div_lim[d] * dot_a[d]);
+          cand_b[d] = need_b && dot_b[d] > 0.0 && !(eb.hdsq > div_lim[d] * dot_b[d]);
+          any_cand = any_cand || cand_a[d] || cand_b[d];
+        }
         if (wave_any(any_cand)) {
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            if (cand[d]) {
-              const double dtv = e.hdsq / dot[d];
-              if (dtv != dtv) {
-                fb[d] = true;
-              } else if (dot[d] > 1e-10 && dtv > 0.0 &&
-                         (dtv < best[d] || (dtv == best[d] && e.idx < bidx[d]))) {
-                second[d] = fmin(second[d], best[d]);
-                best[d] = dtv;
-                bidx[d] = e.idx;
-                bpos[d] = s;
-                // the minimum's scale and flags, for the verification rule below
-                bscale[d] = fabs(e.d0) + sqrt(e.d1 * e.d1 + e.d2 * e.d2 + e.d3 * e.d3);
-                bflags[d] = e.flags;
-              } else {
-                second[d] = fmin(second[d], dtv);
-              }
-              stop_lim[d] = best[d] * ((1.0 + 1e-4) * (1.0 + 1e-6));
-              div_lim[d] = fmax(best[d], second[d]) * (1.0 + 1e-12);
-            }
+            if (cand_a[d]) update(ea, s, dot_a[d], d);
+            if (cand_b[d]) update(eb, s + 1, dot_b[d], d);
           }
         }
         return true;
@@ -420,11 +434,18 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       {
         int s = 0;
         bool more = true;
-        for (; more && s < n_lds; ++s) {
-          const RotCand e = cand_lds[s];
-          more = step1(e, s);
+        for (; more && s < n_lds; s += 2) {
+          const bool has_b = s + 1 < n_lds;
+          const RotCand ea = cand_lds[s];
+          const RotCand eb = cand_lds[has_b ? s + 1 : s];
+          more = step1(ea, eb, s, has_b);
         }
-        for (; more && s < n_cand; ++s) more = step1(rotate_entry(list[s], R, identity), s);
+        for (; more && s < n_cand; s += 2) {
+          const bool has_b = s + 1 < n_cand;
+          const RotCand ea = rotate_entry(list[s], R, identity);
+          const RotCand eb = rotate_entry(list[has_b ? s + 1 : s], R, identity);
+          more = step1(ea, eb, s, has_b);
+        }
       }
 
       // Pass 2: verify each minimum against every other bubble c with the
