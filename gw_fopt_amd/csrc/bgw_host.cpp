@@ -572,6 +572,9 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   const size_t n_k = q.k.size();
   const size_t n_sets = sc.n_sets;
   if (q.n_w > 1024) throw CalcError("n_w > 1024 is not supported by the GPU kernels");
+  if (q.n_t >= static_cast<size_t>(bgw::kSortStepMax))
+    throw CalcError("n_t >= " + std::to_string(bgw::kSortStepMax) +
+                    " is not supported by the GPU kernels");
   ensure_static(E);
   hipStream_t st = E.dev.stream;
 
@@ -688,6 +691,7 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   ga.n_phi_pow2 = static_cast<int>(res.n_phi_pow2);
   ga.tau = E.d_tau.as<double>();
   ga.perm = E.d_perm.as<uint16_t>();
+  ga.n_t = static_cast<int>(q.n_t);
 
   bgw::SpecArgs& sa = Lc.sa;
   sa.real = ga.real;
@@ -1126,10 +1130,11 @@ uint64_t key_bits(double x) {
 }
 
 void key_args(std::ostringstream& k, const bgw::GeoArgs& a) {
-  static_assert(sizeof(bgw::GeoArgs) == 144, "GeoArgs changed: list the new field in key_args");
+  static_assert(sizeof(bgw::GeoArgs) == 152, "GeoArgs changed: list the new field in key_args");
   k << "|G" << a.cand << ',' << a.inv << ',' << a.real << ',' << a.items << ',' << a.rot << ','
     << a.identity << ',' << a.sin_t << ',' << a.cos_t << ',' << a.cos_p << ',' << a.sin_p << ','
     << a.n_c << ',' << a.n_phi << ',' << a.n_phi_pow2 << ',' << a.tau << ',' << a.perm << ','
+    << a.n_t << ','
     << a.first_out << ',' << a.dtab_out << ',' << a.item0 << ',' << a.rings_per_block << ','
     << a.n_split;
 }
@@ -1163,7 +1168,7 @@ void key_args(std::ostringstream& k, const bgw::MatArgs& a) {
 }
 
 void key_args(std::ostringstream& k, const bgw::EnsArgs& a) {
-  static_assert(sizeof(bgw::EnsArgs) == 432, "EnsArgs changed: list the new field in key_args");
+  static_assert(sizeof(bgw::EnsArgs) == 440, "EnsArgs changed: list the new field in key_args");
   key_args(k, a.geo);
   key_args(k, a.spec);
   k << "|E" << a.unit_group << ',' << a.group_item_off << ',' << a.group_r << ',' << a.n_units

@@ -11,6 +11,10 @@
 
 namespace bgw {
 
+// time steps of a grid at most (exclusive): the geometry orders each ring's
+// directions by the step at which they have collided, in 21 bits
+constexpr int kSortStepMax = (1 << 21) - 1;
+
 struct GeoArgs {
   const CandEntry* cand;
   const int32_t* inv;
@@ -23,8 +27,10 @@ struct GeoArgs {
   const double* cos_p;    // (n_phi)
   const double* sin_p;    // (n_phi)
   int n_c, n_phi, n_phi_pow2;
-  double* tau;            // (n_items, n_c * n_phi), each ring sorted ascending
-  uint16_t* perm;         // (n_items, n_c * n_phi), phi index of each sorted entry
+  double* tau;            // (n_items, n_c * n_phi), each ring's delays ordered by the first
+                          //   time step at which the direction has collided (n_t > 0)
+  uint16_t* perm;         // (n_items, n_c * n_phi), phi index of each ordered entry
+  int n_t;                // time steps of the item's grid (0: rings in phi order)
   int32_t* first_out;     // optional (n_items, n_c * n_phi), unsorted
   double* dtab_out;       // optional (n_items, n_c * n_phi), unsorted
   int item0;              // first item of the launch; tau / perm row b is item item0 + b

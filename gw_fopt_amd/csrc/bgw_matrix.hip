@@ -59,37 +59,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // step (a ring of n_t steps carries at most n_t / 64 of them)
 constexpr int kMxAnchorSteps = 16;
 
-__device__ __forceinline__ double dta_at(const RealInfo& ri, int k, double t_a) {
-  return (ri.t_begin + ri.t_step * static_cast<double>(k)) - t_a;
-}
-
-// First k in [0, n_t] with t_k - t_a >= tau (n_t if none), t_k = t_begin +
-// t_step k as the other forms compute it.  t_k - t_a is non-decreasing in k,
-// so #{tau_r <= t_k - t_a} = #{first_step(tau_r) <= k}.  An estimate from the
-// grid, checked and corrected step by step; a binary search where the
-// estimate is far off.
-__device__ __forceinline__ int first_step_reaching(const RealInfo& ri, int n_t, double t_a,
-                                                   double tau) {
-  if (!(tau < kInf)) return n_t;
-  int k = n_t;
-  if (ri.t_step > 0.0) {
-    const double est = (t_a + tau - ri.t_begin) / ri.t_step;
-    if (est < static_cast<double>(n_t)) k = est > 0.0 ? static_cast<int>(est) : 0;
-  }
-  for (int it = 0; it < 4; ++it) {
-    if (k < n_t && dta_at(ri, k, t_a) < tau) ++k;
-    else if (k > 0 && dta_at(ri, k - 1, t_a) >= tau) --k;
-    else return k;
-  }
-  int lo = 0, hi = n_t;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (dta_at(ri, mid, t_a) >= tau) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
-}
-
 __device__ __forceinline__ int first_live_step(const RealInfo& ri, int n_t, double t_a) {
   int lo = 0, hi = n_t;
   while (lo < hi) {

@@ -136,12 +136,58 @@ __device__ __forceinline__ double xor_lanes(double v, int m) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
+__device__ __forceinline__ double dta_at(const RealInfo& ri, int k, double t_a) {
+  return (ri.t_begin + ri.t_step * static_cast<double>(k)) - t_a;
+}
+
+// First k in [0, n_t] with t_k - t_a >= tau (n_t if none), t_k = t_begin +
+// t_step k as the other forms compute it.  t_k - t_a is non-decreasing in k,
+// so #{tau_r <= t_k - t_a} = #{first_step(tau_r) <= k}.  An estimate from the
+// grid, checked and corrected step by step; a binary search where the
+// estimate is far off.
+__device__ __forceinline__ int first_step_reaching(const RealInfo& ri, int n_t, double t_a,
+                                                   double tau) {
+  if (!(tau < kInf)) return n_t;
+  int k = n_t;
+  if (ri.t_step > 0.0) {
+    const double est = (t_a + tau - ri.t_begin) / ri.t_step;
+    if (est < static_cast<double>(n_t)) k = est > 0.0 ? static_cast<int>(est) : 0;
+  }
+  for (int it = 0; it < 4; ++it) {
+    if (k < n_t && dta_at(ri, k, t_a) < tau) ++k;
+    else if (k > 0 && dta_at(ri, k - 1, t_a) >= tau) --k;
+    else return k;
+  }
+  int lo = 0, hi = n_t;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (dta_at(ri, mid, t_a) >= tau) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+
+// Ring slots are sorted by a 31-bit code: the first time step at which the
+// direction has collided (first_step_reaching; n_t when it never does within
+// the grid, 0 for every direction when there is no grid) above the phi index.
+// The directions collided at step k are then exactly a prefix of the ring for
+// every k, which is all the phase sums read from the order; padding slots
+// (phi index >= n_phi) come after every direction.
+constexpr int kSortPhiBits = 10;  // phi index < kSortCap
+__device__ __forceinline__ int32_t ring_sort_code(int n_t, const RealInfo& ri, double t_a,
+                                                  double tau, int jd, int n_phi) {
+  if (jd >= n_phi) return (kSortStepMax << kSortPhiBits) | jd;
+  const int step = n_t > 0 ? first_step_reaching(ri, n_t, t_a, tau) : 0;
+  return (step << kSortPhiBits) | jd;
+}
+
 // Bitonic sort of the P slots of one ring held by one wavefront: lane l
 // holds ring-local elements l + 64 r (r < P / 64) when P >= 64, or element
 // l % P of ring l / P when P < 64.  Partners less than 64 apart are exchanged
 // across lanes (up to 8 apart by DPP moves, 16 and 32 apart by shuffles),
 // farther ones inside a lane; no LDS traffic and no workgroup barrier.  Keys
-// are (delay, phi index) pairs, all distinct.
+// are ring_sort_code values (one 32-bit compare per exchange), all distinct.
 template <int P>
 __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, int gcount, int i0,
                              int n_phi, double* tau_out, uint16_t* perm_out) {
@@ -155,13 +201,11 @@ __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, in
   for (int rb = wave * RPW; rb < gcount; rb += n_waves * RPW) {
     const int rr = rb + rsub;
     const bool valid = rr < gcount;
-    double key[EPL];
-    int32_t val[EPL];
+    int32_t key[EPL];
 #pragma unroll
     for (int r = 0; r < EPL; ++r) {
       const int e = e0 + 64 * r;
-      key[r] = valid ? sort_key[rr * P + e] : kInf;
-      val[r] = valid ? sort_val[rr * P + e] : 0x7fffffff - e;
+      key[r] = valid ? sort_val[rr * P + e] : 0x7fffffff - e;
     }
 #pragma unroll
     for (int k = 2; k <= P; k <<= 1) {
@@ -174,14 +218,10 @@ __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, in
             if (rp > r) {
               const int e = e0 + 64 * r;
               const bool up = (e & k) == 0;
-              const bool greater = key[r] > key[rp] || (key[r] == key[rp] && val[r] > val[rp]);
-              if (greater == up) {
-                const double tk = key[r];
+              if ((key[r] > key[rp]) == up) {
+                const int32_t t = key[r];
                 key[r] = key[rp];
-                key[rp] = tk;
-                const int32_t tv = val[r];
-                val[r] = val[rp];
-                val[rp] = tv;
+                key[rp] = t;
               }
             }
           }
@@ -189,17 +229,12 @@ __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, in
 #pragma unroll
           for (int r = 0; r < EPL; ++r) {
             const int e = e0 + 64 * r;
-            const double pk = j <= 8 ? xor_lanes(key[r], j) : __shfl_xor(key[r], j, 64);
-            const int32_t pv = j <= 8 ? xor_lanes(val[r], j) : __shfl_xor(val[r], j, 64);
+            const int32_t pk = j <= 8 ? xor_lanes(key[r], j) : __shfl_xor(key[r], j, 64);
             const bool lower = (e & j) == 0;
             const bool up = (e & k) == 0;
-            const bool mine_greater = key[r] > pk || (key[r] == pk && val[r] > pv);
+            const bool mine_greater = key[r] > pk;
             // the lower element of the pair keeps the minimum when ascending
-            const bool take = (lower == up) ? mine_greater : !mine_greater;
-            if (take) {
-              key[r] = pk;
-              val[r] = pv;
-            }
+            if ((lower == up) ? mine_greater : !mine_greater) key[r] = pk;
           }
         }
       }
@@ -209,8 +244,9 @@ __device__ void sort_rings_p(const double* sort_key, const int32_t* sort_val, in
       for (int r = 0; r < EPL; ++r) {
         const int e = e0 + 64 * r;
         if (e < n_phi) {
-          tau_out[static_cast<int64_t>(i0 + rr) * n_phi + e] = key[r];
-          perm_out[static_cast<int64_t>(i0 + rr) * n_phi + e] = static_cast<uint16_t>(val[r]);
+          const int jd = key[r] & ((1 << kSortPhiBits) - 1);
+          tau_out[static_cast<int64_t>(i0 + rr) * n_phi + e] = sort_key[rr * P + jd];
+          perm_out[static_cast<int64_t>(i0 + rr) * n_phi + e] = static_cast<uint16_t>(jd);
         }
       }
     }
@@ -230,14 +266,10 @@ __device__ void sort_rings_lds(int P, double* sort_key, int32_t* sort_val, int g
           // direction from the index within the ring's segment, so every
           // segment ends ascending (k == P merges the whole segment)
           const bool up = ((e & (P - 1)) & k) == 0;
-          const double ke = sort_key[e], kp = sort_key[p];
-          const bool greater = ke > kp || (ke == kp && sort_val[e] > sort_val[p]);
-          if (greater == up) {
-            sort_key[e] = kp;
-            sort_key[p] = ke;
-            const int32_t t = sort_val[e];
-            sort_val[e] = sort_val[p];
-            sort_val[p] = t;
+          const int32_t ke = sort_val[e], kp = sort_val[p];
+          if ((ke > kp) == up) {
+            sort_val[e] = kp;
+            sort_val[p] = ke;
           }
         }
       }
@@ -245,11 +277,12 @@ __device__ void sort_rings_lds(int P, double* sort_key, int32_t* sort_val, int g
     }
   }
   for (int e = threadIdx.x; e < nslot; e += blockDim.x) {
-    const int ring = i0 + e / P;
+    const int rr = e / P;
     const int r = e % P;
     if (r < n_phi) {
-      tau_out[static_cast<int64_t>(ring) * n_phi + r] = sort_key[e];
-      perm_out[static_cast<int64_t>(ring) * n_phi + r] = static_cast<uint16_t>(sort_val[e]);
+      const int jd = sort_val[e] & ((1 << kSortPhiBits) - 1);
+      tau_out[static_cast<int64_t>(i0 + rr) * n_phi + r] = sort_key[rr * P + jd];
+      perm_out[static_cast<int64_t>(i0 + rr) * n_phi + r] = static_cast<uint16_t>(jd);
     }
   }
 }
@@ -537,15 +570,13 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
         }
         if (in_group[d]) {
           sort_key[slot[d]] = tau;
-          sort_val[slot[d]] = jd[d];
+          sort_val[slot[d]] = ring_sort_code(g.n_t, ri, it.t_a, tau, jd[d], n_phi);
         }
       }
     }
     __syncthreads();
-    // Sort every ring's P slots ascending in (delay, phi index), one ring per
-    // wavefront in registers (P <= 128).  Padding slots carry phi index >=
-    // n_phi, so they sort after the directions that never collide (delay +inf
-    // as well).
+    // Sort every ring's P slots by ring_sort_code, one ring per wavefront in
+    // registers (P <= 128), the delays following their phi indices.
     sort_rings<SP>(P, sort_key, sort_val, gcount, i0, n_phi, tau_out, perm_out);
     __syncthreads();
   }
