@@ -401,7 +401,9 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       for (int d = 0; d < D; ++d) {
         best[d] = kInf;
         second[d] = kInf;
-        stop_lim[d] = kInf;
+        // NaN for inactive lanes: key <= stop_lim is then false, so that the
+        // exit ballot is one comparison
+        stop_lim[d] = act[d] ? kInf : __builtin_nan("");
         div_lim[d] = kInf;
         bscale[d] = 0.0;
         bidx[d] = 0x7fffffff;
@@ -438,17 +440,18 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
         div_lim[d] = fmax(best[d], second[d]) * (1.0 + 1e-12);
       };
       auto step1 = [&](const RotCand& ea, const RotCand& eb, int s, bool has_b) -> bool {
-        bool need_a[D], cand_a[D], cand_b[D], any_need = false, any_cand = false;
+        bool need_a[D], cand_a[D], cand_b[D], any_cand = false;
         double dot_a[D], dot_b[D];
+        uint64_t need_mask = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          need_a[d] = act[d] && !(ea.key > stop_lim[d]);
-          any_need = any_need || need_a[d];
+          need_a[d] = ea.key <= stop_lim[d];
+          need_mask |= __builtin_amdgcn_ballot_w64(need_a[d]);
         }
-        if (!wave_any(any_need)) return false;
+        if (!need_mask) return false;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          const bool need_b = has_b && act[d] && !(eb.key > stop_lim[d]);
+          const bool need_b = has_b && eb.key <= stop_lim[d];
           dot_a[d] = lscalar(ea.d0, ea.d1, ea.d2, ea.d3, 1.0, x1[d], x2[d], x3[d]);
           dot_b[d] = lscalar(eb.d0, eb.d1, eb.d2, eb.d3, 1.0, x1[d], x2[d], x3[d]);
           cand_a[d] = need_a[d] && dot_a[d] > 0.0 && !(ea.hdsq > div_lim[d] * dot_a[d]);
@@ -467,11 +470,18 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
       {
         int s = 0;
         bool more = true;
+        // the staged list through a pointer held in a VGPR (an opaque zero
+        // offset), so that both candidates of a pair are read at immediate
+        // offsets from one address that advances by one add per pair; the
+        // second read stays inside the array (s + 1 < kGeoLdsCand) and is
+        // unused past the staged candidates
+        int vzero;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
+        const RotCand* cl = cand_lds + vzero;
         for (; more && s < n_lds; s += 2) {
-          const bool has_b = s + 1 < n_lds;
-          const RotCand ea = cand_lds[s];
-          const RotCand eb = cand_lds[has_b ? s + 1 : s];
-          more = step1(ea, eb, s, has_b);
+          const RotCand ea = cl[s];
+          const RotCand eb = cl[s + 1];
+          more = step1(ea, eb, s, s + 1 < n_lds);
         }
         for (; more && s < n_cand; s += 2) {
           const bool has_b = s + 1 < n_cand;
