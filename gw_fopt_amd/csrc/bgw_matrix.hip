@@ -213,6 +213,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const Item it0 = g.items[m.tiles[static_cast<int64_t>(tile) * ni]];
   const RealInfo ri = g.real[it0.r];
   const double t_a = it0.t_a;
+  const double inv_step = ri.t_step > 0.0 ? 1.0 / ri.t_step : 0.0;
   const double* ttile = m.tt + static_cast<int64_t>(tile) * n_t * TW;  // the tile's time tables
 
   // (1) tables: lane (j, seg) takes directions [seg SL, seg SL + SL) of item j
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
         if (r >= r_hi) break;
         const double tau = tb4[u], gp = gp4[u], gx = gx4[u];
         if (r == 0) tau0_s[j] = tau;
-        kap[j * L.KS + r] = static_cast<uint16_t>(first_step_reaching(ri, n_t, t_a, tau));
+        kap[j * L.KS + r] = static_cast<uint16_t>(first_step_reaching(ri, n_t, t_a, tau, inv_step));
         T[L.off(r)] = gp;
         T[L.off(r) + 1] = gx;
         sum[0] += gp;

@@ -143,14 +143,15 @@ __device__ __forceinline__ double dta_at(const RealInfo& ri, int k, double t_a) 
 // First k in [0, n_t] with t_k - t_a >= tau (n_t if none), t_k = t_begin +
 // t_step k as the other forms compute it.  t_k - t_a is non-decreasing in k,
 // so #{tau_r <= t_k - t_a} = #{first_step(tau_r) <= k}.  An estimate from the
-// grid, checked and corrected step by step; a binary search where the
-// estimate is far off.
+// grid (inv_step = 1 / t_step, or 0: the estimate only picks where the exact
+// comparisons start), checked and corrected step by step; a binary search
+// where the estimate is far off.
 __device__ __forceinline__ int first_step_reaching(const RealInfo& ri, int n_t, double t_a,
-                                                   double tau) {
+                                                   double tau, double inv_step) {
   if (!(tau < kInf)) return n_t;
   int k = n_t;
   if (ri.t_step > 0.0) {
-    const double est = (t_a + tau - ri.t_begin) / ri.t_step;
+    const double est = (t_a + tau - ri.t_begin) * inv_step;
     if (est < static_cast<double>(n_t)) k = est > 0.0 ? static_cast<int>(est) : 0;
   }
   for (int it = 0; it < 4; ++it) {
@@ -176,9 +177,10 @@ __device__ __forceinline__ int first_step_reaching(const RealInfo& ri, int n_t, 
 // (phi index >= n_phi) come after every direction.
 constexpr int kSortPhiBits = 10;  // phi index < kSortCap
 __device__ __forceinline__ int32_t ring_sort_code(int n_t, const RealInfo& ri, double t_a,
-                                                  double tau, int jd, int n_phi) {
+                                                  double tau, double inv_step, int jd,
+                                                  int n_phi) {
   if (jd >= n_phi) return (kSortStepMax << kSortPhiBits) | jd;
-  const int step = n_t > 0 ? first_step_reaching(ri, n_t, t_a, tau) : 0;
+  const int step = n_t > 0 ? first_step_reaching(ri, n_t, t_a, tau, inv_step) : 0;
   return (step << kSortPhiBits) | jd;
 }
 
@@ -341,6 +343,7 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
   const int n_phi = g.n_phi;
   const int P = g.n_phi_pow2;
   const int G = kSortCap / P;  // rings per sort group
+  const double inv_step = ri.t_step > 0.0 ? 1.0 / ri.t_step : 0.0;
   const int64_t n_dir = static_cast<int64_t>(g.n_c) * n_phi;
 
   // Each wavefront takes kDirPerLane neighbouring patches of ph rings x pw
@@ -522,24 +525,35 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
 #pragma unroll
         for (int d = 0; d < D; ++d)
           b0[d] = p2[d] ? fetch_cand(bpos[d], cand_lds, n_lds, list, R, identity) : RotCand{};
+        // the verification limit of each lane, NaN where there is nothing
+        // (more) to verify, so that the exit ballot is one comparison
+        double lim2[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) lim2[d] = p2[d] ? thr[d] : __builtin_nan("");
         auto step2 = [&](const RotCand& e, int s) -> bool {
-          bool need[D], any_need = false;
+          bool need[D];
+          uint64_t need_mask = 0;
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            need[d] = p2[d] && ok[d] && !(e.key > thr[d]);
-            any_need = any_need || need[d];
+            need[d] = e.key <= lim2[d];
+            need_mask |= __builtin_amdgcn_ballot_w64(need[d]);
           }
-          if (!wave_any(any_need)) return false;
+          if (!need_mask) return false;
 #pragma unroll
           for (int d = 0; d < D; ++d)
-            if (need[d] && s != bpos[d] && !collision_point_ok(b0[d], e, x1[d], x2[d], x3[d]))
+            if (need[d] && s != bpos[d] && !collision_point_ok(b0[d], e, x1[d], x2[d], x3[d])) {
               ok[d] = false;
+              lim2[d] = __builtin_nan("");
+            }
           return true;
         };
         int s = 0;
         bool more = true;
+        int vzero;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
+        const RotCand* cl = cand_lds + vzero;  // one VGPR address, as in pass 1
         for (; more && s < n_lds; ++s) {
-          const RotCand e = cand_lds[s];
+          const RotCand e = cl[s];
           more = step2(e, s);
         }
         for (; more && s < n_cand; ++s) more = step2(rotate_entry(list[s], R, identity), s);
@@ -580,7 +594,7 @@ __device__ void geometry_item(const GeoArgs& g, int item_id, double* tau_out,
         }
         if (in_group[d]) {
           sort_key[slot[d]] = tau;
-          sort_val[slot[d]] = ring_sort_code(g.n_t, ri, it.t_a, tau, jd[d], n_phi);
+          sort_val[slot[d]] = ring_sort_code(g.n_t, ri, it.t_a, tau, inv_step, jd[d], n_phi);
         }
       }
     }
