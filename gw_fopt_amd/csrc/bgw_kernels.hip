@@ -49,12 +49,13 @@ namespace bgw {
 // Stage 1: first collisions
 // ---------------------------------------------------------------------------
 // SP: ring slots (n_phi rounded up to a power of two) for a register sort,
-// -1 for the LDS sort of wider rings.  The first-collider walks are latency
-// bound, so the kernel is held to 64 VGPRs for 8 waves per SIMD (8 workgroups
-// of 20 KB fill the CU's 160 KB of LDS); the compiler parks two loop-invariant
-// addresses in scratch, reloaded once per ring group.
+// -1 for the LDS sort of wider rings.  The kernel is held to 72 VGPRs for 7
+// waves per SIMD (7 workgroups of 20 KB of LDS per CU): it is VALU-issue bound
+// once the walks are short, and at 64 VGPRs (8 waves) the register spills'
+// extra instructions cost more than the eighth wave gains (2.51 against
+// 2.49 ms at plain_256; 6 waves 2.53 ms).
 template <int SP>
-__global__ __launch_bounds__(kGeoThreads) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ __launch_bounds__(kGeoThreads) __attribute__((amdgpu_waves_per_eu(7)))
 void geometry_kernel(GeoArgs g) {
   __shared__ RotCand cand_lds[kGeoLdsCand];
   __shared__ double sort_key[kSortCap];
