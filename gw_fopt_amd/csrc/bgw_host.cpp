@@ -770,7 +770,7 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
   const size_t ncol = 16 * static_cast<size_t>(ma.col_tiles);
   std::vector<int32_t> tiles;
   std::vector<std::pair<double, int32_t>> cost;
-  std::vector<int32_t> kbase_of;
+  std::vector<int32_t> kbase_of, k_a_of;
   const size_t n_real = E.real.size();
   for (size_t r = 0; r < n_real; ++r) {
     const size_t n_sel = q.bubbles_of_real[r].size();
@@ -792,6 +792,7 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
           else lo = mid + 1;
         }
         kbase_of.push_back(lo & ~3);
+        k_a_of.push_back(lo);
       }
   }
   std::stable_sort(cost.begin(), cost.end(),
@@ -808,6 +809,7 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
     ordered.insert(ordered.end(), tiles.begin() + static_cast<size_t>(c) * ma.ni,
                    tiles.begin() + static_cast<size_t>(c + 1) * ma.ni);
     info[t].kbase = kbase_of[c];
+    info[t].k_a = k_a_of[c];
     info[t].len = (static_cast<int32_t>(q.n_t) - kbase_of[c] + 3) & ~3;
     info[t].voff = voff;
     voff += static_cast<int64_t>(sb.n_c) * info[t].len * static_cast<int64_t>(ncol);

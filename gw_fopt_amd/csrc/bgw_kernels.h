@@ -127,6 +127,7 @@ struct MxTileInfo {
   int64_t voff;           // first double of the tile's weight rows
   int32_t kbase;          // step of row 0: the first weighted step of the bubble, rounded down to 4
   int32_t len;            // rows per ring: n_t - kbase rounded up to 4
+  int32_t k_a;            // the first step with t_k - t_a > 0 (n_t if none)
 };
 struct MatArgs {
   SpecArgs s;             // the phase-sum arguments (no damping; n_t <= 65535)

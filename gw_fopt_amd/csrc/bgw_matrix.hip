@@ -59,16 +59,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // step (a ring of n_t steps carries at most n_t / 64 of them)
 constexpr int kMxAnchorSteps = 16;
 
-__device__ __forceinline__ int first_live_step(const RealInfo& ri, int n_t, double t_a) {
-  int lo = 0, hi = n_t;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (dta_at(ri, mid, t_a) > 0.0) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
-}
-
 // number of entries of the ascending row[0..n) that are <= k
 __device__ __forceinline__ int count_steps_le(const uint16_t* row, int n, int k) {
   int lo = 0, hi = n;
@@ -315,22 +305,26 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   // exactly when the ring factor vanishes; a ring whose uncollided sum is
   // rounding noise contributes from its first collision on, and nothing if it
   // never collides
-  if (tid < nj) {
-    const double* Tj = tab + static_cast<size_t>(tid) * L.TS;
-    const bool noise = fabs(Tj[0]) <= 1e-12 * n_phi && fabs(Tj[1]) <= 1e-12 * n_phi;
-    const bool drop_early = g.skip_uncollided && noise && !g.has_damping;
-    const int k_a = first_live_step(ri, n_t, t_a);
-    int lo = k_a;
-    if (item_s[tid] < 0 || rf_s[tid] == 0.0) lo = n_t;
-    else if (drop_early) lo = tau0_s[tid] < kInf ? max(k_a, static_cast<int>(kap[tid * L.KS])) : n_t;
-    tlo_s[tid] = lo;
-  }
-  __syncthreads();
-  if (tid == 0) {
+  // (the tile's first live step k_a comes with its tile info), by the first
+  // wavefront, which also takes the minimum over the items
+  if (tid < 64) {
     int lo = n_t;
-    for (int jj = 0; jj < nj; ++jj) lo = min(lo, tlo_s[jj]);
-    m.ctlo[(static_cast<int64_t>(tile) * g.n_c + i) * 2 + ct] = lo;
-    lo_s = lo;
+    if (tid < nj) {
+      const double* Tj = tab + static_cast<size_t>(tid) * L.TS;
+      const bool noise = fabs(Tj[0]) <= 1e-12 * n_phi && fabs(Tj[1]) <= 1e-12 * n_phi;
+      const bool drop_early = g.skip_uncollided && noise && !g.has_damping;
+      const int k_a = info.k_a;
+      lo = k_a;
+      if (item_s[tid] < 0 || rf_s[tid] == 0.0) lo = n_t;
+      else if (drop_early) lo = tau0_s[tid] < kInf ? max(k_a, static_cast<int>(kap[tid * L.KS])) : n_t;
+      tlo_s[tid] = lo;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off, 64));
+    if (tid == 0) {
+      m.ctlo[(static_cast<int64_t>(tile) * g.n_c + i) * 2 + ct] = lo;
+      lo_s = lo;
+    }
   }
   __syncthreads();
   // (4) this column tile's weight rows from the first one the products read:
