@@ -528,7 +528,20 @@ struct Request {
   // run(): also the k-hat mean of S per realisation and the mean and spread of
   // those over the realisations (an ensemble), inside the timed region
   bool ensemble_stats = false;
+  // setup(): the per-item phase-sum launch order (false: the caller orders the
+  // items itself, as the batched ensembles do)
+  bool item_order = true;
 };
+
+// Indices [0, n) ordered by cost descending, ties in index order (the order of
+// a stable sort on cost), sorted as contiguous (cost, index) pairs.
+template <typename CostAt>
+void order_by_cost(int32_t* idx, size_t n, CostAt cost_at) {
+  std::vector<std::pair<double, int32_t>> key(n);
+  for (size_t i = 0; i < n; ++i) key[i] = {-cost_at(idx[i]), idx[i]};
+  std::sort(key.begin(), key.end());
+  for (size_t i = 0; i < n; ++i) idx[i] = key[i].second;
+}
 
 struct Result {
   std::vector<double> c;  // (n_groups, 2, n_sets, n_w, 2)
@@ -620,12 +633,14 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   }
   const size_t n_items = items.size();
   // launch order of the per-item phase-sum kernels: most expensive items first
-  // (the matrix form orders its tiles itself)
-  std::vector<int32_t> order(n_items);
-  std::iota(order.begin(), order.end(), 0);
-  if (q.mode != 3)
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+  // (the matrix form orders its tiles itself and reads no item order)
+  const bool item_order = q.item_order && q.mode != 3;
+  std::vector<int32_t> order;
+  if (item_order) {
+    order.resize(n_items);
+    std::iota(order.begin(), order.end(), 0);
+    order_by_cost(order.data(), n_items, [&](int32_t i) { return cost[i]; });
+  }
 
   const size_t n_dir = res.n_c * res.n_phi;
   const size_t nq = n_sets * q.n_w;
@@ -638,7 +653,7 @@ void setup(Engine& E, const Request& q, Launch& Lc) {
   Lc.real = real;
   upload(E.d_real, real, st);
   upload(E.d_items, items, st);
-  upload(E.d_order, order, st);
+  if (item_order) upload(E.d_order, order, st);
   upload(E.d_rot, rot, st);
   upload(E.d_sin_t, res.sin_t, st);
   upload(E.d_cos_t, res.cos_t, st);
@@ -1201,6 +1216,7 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
   const bool pipelined = tiles.items_per_chunk > 0;
   Request qq = q;
   qq.item_buffers = false;
+  qq.item_order = false;  // the phase-sum order is built per chunk below
   qq.tile_threads = static_cast<int>(tiles.threads);
   qq.tile_rb = bgw::resolve_block_steps(static_cast<int>(tiles.block_steps), q.mode);
   Launch Lc;
@@ -1305,8 +1321,8 @@ EnsResult run_ensemble(Engine& E, const Request& q, const BgwTileConfig& tiles,
     spec_order.resize(items_u.size());
     for (const auto& c : chunks) {
       std::iota(spec_order.begin() + c.item0, spec_order.begin() + c.item0 + c.n_items, c.item0);
-      std::stable_sort(spec_order.begin() + c.item0, spec_order.begin() + c.item0 + c.n_items,
-                       [&](int32_t x, int32_t y) { return cost_u[x] > cost_u[y]; });
+      order_by_cost(spec_order.data() + c.item0, static_cast<size_t>(c.n_items),
+                    [&](int32_t i) { return cost_u[i]; });
     }
     upload(E.d_items, items_u, st);
     upload(E.d_unit_off, unit_off, st);
