@@ -404,7 +404,7 @@ struct Engine {
       d_ring_fac, d_omega, d_coef, d_pows, d_tau, d_perm, d_partial, d_group_off, d_group_r,
       d_c, d_s, d_first, d_dtab, d_gradc, d_gradout, d_iota, d_acc;
   // FP64 matrix form: tiles, their time tables and weight rows
-  DevBuf d_tiles, d_tinfo, d_tt, d_ctlo, d_vbuf;
+  DevBuf d_tiles, d_tinfo, d_tt, d_ctlo, d_vbuf, d_mxphase;
   // batched ensemble: work units, queue counter, statistics, captured graph
   DevBuf d_units, d_unit_off, d_counter, d_kmean, d_emean, d_estd;
   hipGraphExec_t graph = nullptr;
@@ -417,7 +417,7 @@ struct Engine {
             &d_cos_p, &d_sin_p, &d_gcos, &d_gsin, &d_ring_c, &d_ring_fac, &d_omega, &d_coef,
             &d_pows, &d_tau, &d_perm, &d_partial, &d_group_off, &d_group_r, &d_c, &d_s,
             &d_first, &d_dtab, &d_gradc, &d_gradout, &d_iota, &d_acc, &d_tiles, &d_tinfo,
-            &d_tt, &d_ctlo, &d_vbuf, &d_units, &d_unit_off, &d_counter, &d_kmean,
+            &d_tt, &d_ctlo, &d_vbuf, &d_mxphase, &d_units, &d_unit_off, &d_counter, &d_kmean,
             &d_emean, &d_estd};
   }
   ~Engine() {
@@ -838,6 +838,8 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
   ma.tinfo = E.d_tinfo.as<bgw::MxTileInfo>();
   ma.tt = E.d_tt.as<double>();
   ma.ctlo = E.d_ctlo.as<int32_t>();
+  E.d_mxphase.ensure(std::max<size_t>(1, n_real * sb.n_c * q.n_w * 4) * sizeof(double));
+  ma.phase = E.d_mxphase.as<double>();
   // batches of whole tiles within the row budget (a tile larger than the budget alone)
   std::vector<std::pair<size_t, size_t>> batches;
   size_t max_batch = 1;
@@ -866,6 +868,7 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
 // the products (timed with events unless `timed` is false, as under capture).
 void launch_matrix_plan(Engine& E, const MatrixPlan& P, hipStream_t st, bool timed) {
   bgw::MatArgs ma = P.ma;
+  HIPCHECK(bgw::launch_matrix_phases(ma, static_cast<int>(E.real.size()), st));
   for (size_t b = 0; b < P.batches.size(); ++b) {
     ma.tile0 = static_cast<int>(P.batches[b].first);
     ma.voff0 = P.voff0[b];
@@ -1177,11 +1180,11 @@ void key_args(std::ostringstream& k, const bgw::RedArgs& a) {
 }
 
 void key_args(std::ostringstream& k, const bgw::MatArgs& a) {
-  static_assert(sizeof(bgw::MatArgs) == 296, "MatArgs changed: list the new field in key_args");
+  static_assert(sizeof(bgw::MatArgs) == 304, "MatArgs changed: list the new field in key_args");
   key_args(k, a.s);
   k << "|M" << a.tiles << ',' << a.tinfo << ',' << a.ni << ',' << a.omega_tiles << ','
     << a.col_tiles << ',' << a.tile0 << ',' << a.voff0 << ',' << a.vbuf << ',' << a.tt << ','
-    << a.ctlo;
+    << a.ctlo << ',' << a.phase;
 }
 
 void key_args(std::ostringstream& k, const bgw::EnsArgs& a) {

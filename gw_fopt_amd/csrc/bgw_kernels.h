@@ -142,6 +142,8 @@ struct MatArgs {
   double* tt;             // (n_tiles, n_t, 1 + n_sets n_terms) time tables
   int32_t* ctlo;          // (n_tiles, n_c, 2) first weighted step of each column tile
                           //   (written by the weights kernel, one workgroup per column tile)
+  double* phase;          // (n_real, n_c, n_w, 4) per ring and omega the step factor
+                          //   e^{i w 4 dt (1 - c_i)} and the anchor factor (cos, sin each)
 };
 // n_sets * n_terms of one matrix-form launch
 constexpr int kMaxMatrixTerms = 12;
@@ -153,6 +155,8 @@ size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles);
 // the time tables and weight rows of the launch's tiles, then their matrix products
 hipError_t launch_matrix_weights(const MatArgs& a, int n_tiles, hipStream_t s);
 hipError_t launch_matrix_products(const MatArgs& a, int n_tiles, hipStream_t s);
+// the phase factors of every realisation's rings and omegas (MatArgs::phase)
+hipError_t launch_matrix_phases(const MatArgs& a, int n_real, hipStream_t s);
 
 // Default launch shape of the phase sums (0 in SpecArgs::mixed_block /
 // threads): time steps per factorised block, 8 for the FP64 blocked form and

@@ -17,7 +17,7 @@
 // the four FP64 multiply-adds of a complex-by-real product, issued by the
 // matrix cores.
 //
-// Three kernels per batch of tiles:
+// Three kernels per batch of tiles (and one per call):
 //  * mx_time_kernel: the tile's time tables (K dt w_t cutoff(t) (t - t_a)^3 and
 //    coef_q (t - t_a)^-p_q per step), shared by its rings.
 //  * mx_weights_kernel, one workgroup per (tile, ring): the items' per-ring
@@ -28,6 +28,8 @@
 //    step), with the arithmetic of the other forms' weights
 //    (compute_b_integral, :481-586, folded with the quadrature weights of
 //    :588-664, 775-873).  Latency-bound work, run at full occupancy.
+//  * mx_phase_kernel, once per call: per realisation, ring and omega the step
+//    and anchor factors below (the same values for every tile).
 //  * mx_gemm_kernel, one workgroup per (tile, block of omegas): each wavefront
 //    streams the weight rows of every ring as the B operand (loaded four
 //    matrix steps ahead, from a pointer with immediate offsets) and generates
@@ -390,6 +392,30 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   }
 }
 
+// Per (realisation, ring, omega): the step factor e^{i w 4 dt (1 - c_i)} and
+// the anchor factor e^{i w 4 kMxAnchorSteps dt (1 - c_i)} of the products
+// kernel, once per call instead of once per tile and ring.
+__global__ __launch_bounds__(256) void mx_phase_kernel(MatArgs m, int n_real) {
+  const SpecArgs& g = m.s;
+  const int64_t n = static_cast<int64_t>(n_real) * g.n_c * g.n_w;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int wi = static_cast<int>(e % g.n_w);
+  const int i = static_cast<int>((e / g.n_w) % g.n_c);
+  const int r = static_cast<int>(e / (static_cast<int64_t>(g.n_w) * g.n_c));
+  const RealInfo ri = g.real[r];
+  const double om = g.omega[ri.w_off + wi];
+  const double ai = 1.0 - g.ring_c[i];
+  double* p = m.phase + e * 4;
+  double sr, si, fr, fi;
+  sincos_f64(om * (4.0 * ri.t_step * ai), &si, &sr);
+  sincos_f64(om * ((4.0 * kMxAnchorSteps) * ri.t_step * ai), &fi, &fr);
+  p[0] = sr;
+  p[1] = si;
+  p[2] = fr;
+  p[3] = fi;
+}
+
 // One (tile, block of omegas): wavefront w takes omega tiles w OT + [0, OT) of
 // the block and every column tile; the weight rows stream from HBM four
 // matrix steps ahead.  ct_lo: first weighted step of each column tile.
@@ -459,10 +485,21 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
     // and the exact phase of matrix step 0
     double sr[OT], si[OT], fr[OT], fi[OT], er[OT], ei[OT];
     const double tk0 = ri.t_begin + ri.t_step * static_cast<double>(k0 + kk);
+    const double* ph = m.phase + (static_cast<int64_t>(it0.r) * g.n_c + i) * g.n_w * 4;
 #pragma unroll
     for (int o = 0; o < OT; ++o) {
-      sincos_f64(om[o] * (4.0 * ri.t_step * ai), &si[o], &sr[o]);
-      sincos_f64(om[o] * ((4.0 * kMxAnchorSteps) * ri.t_step * ai), &fi[o], &fr[o]);
+      const int wi = wbase + o * 16 + (lane & 15);
+      sr[o] = 1.0;
+      si[o] = 0.0;
+      fr[o] = 1.0;
+      fi[o] = 0.0;
+      if (wi < g.n_w) {
+        const double* p = ph + static_cast<int64_t>(wi) * 4;
+        sr[o] = p[0];
+        si[o] = p[1];
+        fr[o] = p[2];
+        fi[o] = p[3];
+      }
       sincos_f64(om[o] * (tk0 * ai + t_a * ci), &ei[o], &er[o]);
     }
     double xr[OT], xi[OT];  // the anchor of the current group of kMxAnchorSteps steps
@@ -606,6 +643,15 @@ hipError_t launch_matrix_weights(const MatArgs& a, int n_tiles, hipStream_t s) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_weights(a, n_tiles, lds, s);
+}
+
+hipError_t launch_matrix_phases(const MatArgs& a, int n_real, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(n_real) * a.s.n_c * a.s.n_w;
+  if (n == 0) return hipSuccess;
+  if ((n + 255) / 256 > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mx_phase_kernel, dim3(static_cast<unsigned int>((n + 255) / 256)), dim3(256),
+                     0, s, a, n_real);
+  return hipGetLastError();
 }
 
 hipError_t launch_matrix_products(const MatArgs& a, int n_tiles, hipStream_t s) {
