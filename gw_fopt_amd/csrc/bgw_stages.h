@@ -1059,8 +1059,13 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
                 double b1 = __builtin_fma(hh[q][s][0], ebi[q], hh[q][s][1] * ebr[q]);
                 double b2 = __builtin_fma(hh[q][s][2], ebr[q], -(hh[q][s][3] * ebi[q]));
                 double b3 = __builtin_fma(hh[q][s][2], ebi[q], hh[q][s][3] * ebr[q]);
+                // E_0 = 1 exactly: its term is one addition per polarisation
+                // (the same rounding as the multiply-add by 1; the product with
+                // Im E_0 = 0 would add a zero)
+                b0 += vv[0][2 * s];
+                b2 += vv[0][2 * s + 1];
 #pragma unroll
-                for (int r = 0; r < RB; ++r) {
+                for (int r = 1; r < RB; ++r) {
                   const double vp = vv[r][2 * s], vx = vv[r][2 * s + 1];
                   b0 = __builtin_fma(vp, static_cast<double>(er[q][r]), b0);
                   b1 = __builtin_fma(vp, static_cast<double>(ei[q][r]), b1);
@@ -1104,15 +1109,19 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
             for (int r = 0; r < RB; ++r)
 #pragma unroll
               for (int c = 0; c < 2 * NS; ++c) vv[r][c] = L.vf[(l0 + r) * 2 * NS + c];
+            // the block sums start from the E_0 = 1 term
             Acc bs[OPL][NS][4];
 #pragma unroll
             for (int q = 0; q < OPL; ++q)
 #pragma unroll
-              for (int s = 0; s < NS; ++s)
+              for (int s = 0; s < NS; ++s) {
+                bs[q][s][0] = vv[0][2 * s];
+                bs[q][s][1] = Acc(0);
+                bs[q][s][2] = vv[0][2 * s + 1];
+                bs[q][s][3] = Acc(0);
+              }
 #pragma unroll
-                for (int c = 0; c < 4; ++c) bs[q][s][c] = Acc(0);
-#pragma unroll
-            for (int r = 0; r < RB; ++r) {
+            for (int r = 1; r < RB; ++r) {
 #pragma unroll
               for (int s = 0; s < NS; ++s) {
                 const Acc vp = vv[r][2 * s];
