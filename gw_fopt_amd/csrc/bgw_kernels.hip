@@ -12,15 +12,16 @@
 //   (a ballot), first to find the minimum-delay candidate, then to verify it.
 //   A lane whose candidate fails verification, or that meets non-finite data,
 //   re-runs the reference's own full scan, so decisions are bit-exact.  Each
-//   ring's delays are then sorted for stage 2 (a bitonic network in registers,
-//   one ring per wavefront; in LDS for rings of more than 128 slots).  A
+//   ring's directions are then ordered for stage 2 by the time step at which
+//   they have collided (a bitonic network on integer codes in registers, one
+//   ring per wavefront; in LDS for rings of more than 128 slots).  A
 //   workgroup takes a range of an item's rings (launch_geometry splits items
 //   so that uneven items balance).
 //
 // Stage 2 (spectrum_kernel): per item, ring by ring, the phi integral
 //   B_pm(t) of compute_b_integral (:481-586) is read off prefix sums over the
-//   sorted delays -- the directions already collided at time t are exactly a
-//   prefix of the sorted list -- and the quadrature weights of the A and C
+//   ordered directions -- the directions already collided at time step t are
+//   exactly a prefix of the ring -- and the quadrature weights of the A and C
 //   integrals (:588-664, 775-873) are folded in.  Lanes own omegas and
 //   accumulate C_pm(omega) = sum_t V_pm(t) exp(i omega psi(t)), with
 //   psi = (t - z_a) - (t - t_a) cos th, in registers:

@@ -20,12 +20,13 @@
 // Three kernels per batch of tiles (and one per call):
 //  * mx_time_kernel: the tile's time tables (K dt w_t cutoff(t) (t - t_a)^3 and
 //    coef_q (t - t_a)^-p_q per step), shared by its rings.
-//  * mx_weights_kernel, one workgroup per (tile, ring): the items' per-ring
-//    tables in LDS -- for every sorted direction the first time index at which
-//    it has collided, the suffix sums of the uncollided weights and the prefix
-//    sums of the collided ones -- then every weight V_i[k, col] of the ring
-//    into HBM, row k of 16 CT doubles (zero before an item's first weighted
-//    step), with the arithmetic of the other forms' weights
+//  * mx_weights_kernel, one workgroup per (tile, ring, column tile): the
+//    items' per-ring tables in LDS -- for every direction in ring order the
+//    first time index at which it has collided, the suffix sums of the
+//    uncollided weights and the prefix sums of the collided ones -- then the
+//    column tile's weights V_i[k, col] of the ring into HBM, row k of 16 CT
+//    doubles (rows from the column tile's first weighted step), with the
+//    arithmetic of the other forms' weights
 //    (compute_b_integral, :481-586, folded with the quadrature weights of
 //    :588-664, 775-873).  Latency-bound work, run at full occupancy.
 //  * mx_phase_kernel, once per call: per realisation, ring and omega the step
@@ -95,7 +96,7 @@ __device__ void wave_scan_strided(double* arr, int n, int stride, bool reverse) 
 // Per-ring tables of a tile in LDS.  Item j's record c (c = 0..n_phi, the
 // number of directions already collided) holds W = 2 + 2 n_sets n_terms
 // doubles: U+, Ux (the uncollided weights w_j cos 2 phi_j, w_j sin 2 phi_j
-// summed over the sorted directions >= c) and, per term q, P+_q, Px_q (the
+// summed over the ordered directions >= c) and, per term q, P+_q, Px_q (the
 // collided weights times tau^p_q summed over the directions < c).  Records of
 // one item are consecutive; items are TS doubles apart (odd, to spread the
 // items over the LDS banks).
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void mx_time_kernel(MatArgs m) {
 // the column tile's first weighted step (ctlo).  Splitting a tile's columns
 // over workgroups halves the LDS of each, so twice as many wavefronts keep
 // this latency-bound work in flight.
-//  * tables: the 32 lanes t = 32 j + g of item j take the sorted directions
+//  * tables: the 32 lanes t = 32 j + g of item j take the ordered directions
 //    [g 2^SLS, (g + 1) 2^SLS); each sums its segment, a 32-lane scan of the
 //    segment sums gives every segment's start, and a second walk writes the
 //    records.

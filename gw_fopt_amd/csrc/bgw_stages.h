@@ -315,7 +315,7 @@ __device__ void sort_rings(int P, double* sort_key, int32_t* sort_val, int gcoun
   }
 }
 
-// One item's first colliders and sorted per-ring delays for rings
+// One item's first colliders and ordered per-ring delays for rings
 // [ring_begin, ring_end) (all of them by default), written to tau_out /
 // perm_out (the item's n_c * n_phi row).  Called by the whole workgroup; the
 // LDS arrays hold kGeoLdsCand candidates and kSortCap sort slots.  Ends
@@ -651,11 +651,14 @@ __device__ __forceinline__ double pow_param(double x, double p) {
   return pow(x, p);
 }
 
-__device__ __forceinline__ int count_le(const double* sorted, int n, double x) {
+// Number of a ring's delays tau[0..n) <= x, for x = t_k - t_a at a grid step
+// k: the ring is ordered by collision step, so the entries <= x are exactly
+// a prefix (those of steps <= k) and a binary search finds its end.
+__device__ __forceinline__ int count_le(const double* tau, int n, double x) {
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (sorted[mid] <= x) lo = mid + 1;
+    if (tau[mid] <= x) lo = mid + 1;
     else hi = mid;
   }
   return lo;
@@ -681,8 +684,8 @@ constexpr int kReanchor = 16;
 // Per-ring tables in LDS, laid out after each other, then the per-item time
 // tables (when they fit) and the per-chunk phase inputs.
 struct RingLds {
-  double* tau;  // [n_phi] sorted delays
-  double* gc;   // [n_phi] w_j cos 2phi_j in sorted order
+  double* tau;  // [n_phi] delays in ring order (by collision step)
+  double* gc;   // [n_phi] w_j cos 2phi_j in ring order
   double* gs;   // [n_phi]
   double* uc;   // [n_phi + 1] suffix sums of gc (uncollided part)
   double* us;   // [n_phi + 1]
