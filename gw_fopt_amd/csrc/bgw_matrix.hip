@@ -20,7 +20,8 @@
 // Three kernels per batch of tiles (and one per call):
 //  * mx_time_kernel: the tile's time tables (K dt w_t cutoff(t) (t - t_a)^3 and
 //    coef_q (t - t_a)^-p_q per step), shared by its rings.
-//  * mx_weights_kernel, one workgroup per (tile, ring, column tile): the
+//  * mx_weights_kernel, one workgroup per (tile, ring, column tile; both
+//    column tiles for rings of at most 64 directions and one set): the
 //    items' per-ring tables in LDS -- for every direction in ring order the
 //    first time index at which it has collided, the suffix sums of the
 //    uncollided weights and the prefix sums of the collided ones -- then the
@@ -107,16 +108,24 @@ __device__ void wave_scan_strided(double* arr, int n, int stride, bool reverse) 
 constexpr int kMxSegLanes = 32;
 constexpr int kMxWeightItems = 256 / kMxSegLanes;  // items of one weights workgroup
 
+// Lanes per item of the weights workgroups: 32, or 16 for one set of one term
+// in two column tiles of rings of at most 64 directions -- then a workgroup
+// holds 16 items, both column tiles of the tile, in the LDS that 8 items of
+// 128 directions take (fewer, fuller workgroups for small grids).
+__host__ __device__ inline int mx_seg_lanes(int n_phi, int n_sets, int n_terms, int col_tiles) {
+  return (n_sets == 1 && n_terms == 1 && col_tiles == 2 && n_phi <= 64) ? 16 : kMxSegLanes;
+}
+
 struct MxLayout {
   int W, TS, KS, SLS;  // record width, item stride (doubles), collided-step row stride
                        // (uint16), log2 of the records per lane segment
   __host__ __device__ int off(int c) const { return c * W + (c >> SLS); }
 };
 
-__host__ __device__ inline MxLayout mx_layout(int n_phi, int n_sets, int n_terms) {
+__host__ __device__ inline MxLayout mx_layout(int n_phi, int n_sets, int n_terms, int seg) {
   MxLayout L;
   L.W = 2 + 2 * n_sets * n_terms;
-  const int sl = (n_phi + kMxSegLanes - 1) / kMxSegLanes;
+  const int sl = (n_phi + seg - 1) / seg;
   L.SLS = 0;
   while ((1 << L.SLS) < sl) ++L.SLS;
   L.TS = L.off(n_phi) + L.W;
@@ -151,27 +160,29 @@ __global__ __launch_bounds__(256) void mx_time_kernel(MatArgs m) {
   }
 }
 
-// One (tile, ring, column tile ct): the tables of the items whose columns
-// meet columns [16 ct, 16 ct + 16) (at most kMxWeightItems), then those 16
-// columns of the ring's weight rows k = kbase .. kbase + len - 1 into HBM, and
-// the column tile's first weighted step (ctlo).  Splitting a tile's columns
+// One (tile, ring, CTW column tiles from ct0): the tables of the items whose
+// columns meet columns [16 ct0, 16 (ct0 + CTW)) (at most 256 / SEG), then
+// those columns of the ring's weight rows k = kbase .. kbase + len - 1 into
+// HBM, and each column tile's first weighted step (ctlo).  With 32 lanes per
+// item a workgroup takes one column tile (CTW = 1): splitting a tile's columns
 // over workgroups halves the LDS of each, so twice as many wavefronts keep
-// this latency-bound work in flight.
-//  * tables: the 32 lanes t = 32 j + g of item j take the ordered directions
-//    [g 2^SLS, (g + 1) 2^SLS); each sums its segment, a 32-lane scan of the
+// this latency-bound work in flight; with 16 lanes (small rings) it takes both.
+//  * tables: the SEG lanes t = SEG j + g of item j take the ordered directions
+//    [g 2^SLS, (g + 1) 2^SLS); each sums its segment, a SEG-lane scan of the
 //    segment sums gives every segment's start, and a second walk writes the
 //    records.
-//  * rows: lane t writes the two columns (+, x) 16 ct + 2 (t % 8) and the
-//    next of rows from the column tile's first weighted step (rounded down to
-//    4; the products read nothing of the column tile before it), 32 rows
-//    apart; the collided count is found once by a binary search and carried
-//    along the lane's rows.
+//  * rows: lane t writes the two columns (+, x) 16 ct0 + 2 (t % CP) and the
+//    next (CP = 8 CTW column pairs) of rows from the workgroup's first
+//    weighted step (rounded down to 4; the products read nothing of a column
+//    tile before its own), 256 / CP rows apart; the collided count is found
+//    once by a binary search and carried along the lane's rows.
 // FAST: one set of one term.
-template <bool FAST>
+template <bool FAST, int SEG>
 __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   extern __shared__ double smem[];
   constexpr int MAXW = FAST ? 4 : 2 + 2 * kMaxMatrixTerms;
-  constexpr int NJ = kMxWeightItems;
+  constexpr int NJ = 256 / SEG;
+  constexpr int CTW = SEG == 16 ? 2 : 1;  // column tiles of the workgroup
   __shared__ int item_s[NJ];
   __shared__ int tlo_s[NJ];
   __shared__ double rf_s[NJ];
@@ -180,7 +191,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const SpecArgs& g = m.s;
   const int tile = m.tile0 + static_cast<int>(blockIdx.x);
   const int i = blockIdx.y;
-  const int ct = blockIdx.z;
+  const int ct0 = static_cast<int>(blockIdx.z) * CTW;
   const int ni = m.ni;
   const int ncol = 16 * m.col_tiles;  // doubles per weight row
   const int nsets = FAST ? 1 : g.n_sets;
@@ -188,14 +199,14 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const int nterms = nsets * n_terms;
   const int n_phi = g.n_phi;
   const int n_t = g.n_t;
-  const MxLayout L = mx_layout(n_phi, nsets, n_terms);
+  const MxLayout L = mx_layout(n_phi, nsets, n_terms, SEG);
   const int W = FAST ? 4 : L.W;
   const int TW = 1 + nterms;
   const MxTileInfo info = m.tinfo[tile];
-  // items j0 .. j1 - 1 of the tile have columns in this column tile
+  // items j0 .. j1 - 1 of the tile have columns in this workgroup's column tiles
   const int cpi = 2 * nsets;
-  const int j0 = (16 * ct) / cpi;
-  const int j1 = min(ni, (16 * ct + 15) / cpi + 1);
+  const int j0 = (16 * ct0) / cpi;
+  const int j1 = min(ni, (16 * (ct0 + CTW) - 1) / cpi + 1);
   const int nj = j1 - j0;
   const int nj_cap = min(ni, NJ);  // the tables' item slots (matrix_lds_bytes)
   double* tab = smem;                                                                  // [nj_cap][TS]
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const double* ttile = m.tt + static_cast<int64_t>(tile) * n_t * TW;  // the tile's time tables
 
   // (1) tables: lane (j, seg) takes directions [seg SL, seg SL + SL) of item j
-  const int j = tid / kMxSegLanes, seg = tid % kMxSegLanes;
+  const int j = tid / SEG, seg = tid % SEG;
   const int SL = 1 << L.SLS;
   const int r_lo = min(n_phi, seg * SL), r_hi = min(n_phi, r_lo + SL);
   const bool jlive = j < nj && item_s[j] >= 0;
@@ -268,11 +279,11 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
     if (a >= W) break;
     double v = sum[a];
 #pragma unroll
-    for (int off = 1; off < kMxSegLanes; off <<= 1) {
-      const double u = __shfl_up(v, off, kMxSegLanes);
+    for (int off = 1; off < SEG; off <<= 1) {
+      const double u = __shfl_up(v, off, SEG);
       if (seg >= off) v += u;
     }
-    total[a] = __shfl(v, kMxSegLanes - 1, kMxSegLanes);
+    total[a] = __shfl(v, SEG - 1, SEG);
     excl[a] = v - sum[a];
   }
   if (j < nj) {
@@ -292,7 +303,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
         T[L.off(r + 1) + a] = pre[a];
       }
     }
-    if (seg == kMxSegLanes - 1) {
+    if (seg == SEG - 1) {
       T[L.off(n_phi)] = 0.0;
       T[L.off(n_phi) + 1] = 0.0;
       for (int a = 2; a < W; ++a) T[a] = 0.0;
@@ -309,9 +320,9 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   // rounding noise contributes from its first collision on, and nothing if it
   // never collides
   // (the tile's first live step k_a comes with its tile info), by the first
-  // wavefront, which also takes the minimum over the items
+  // wavefront, which also takes the minimum over each column tile's items
   if (tid < 64) {
-    int lo = n_t;
+    int lo = n_t, col0 = -1 << 20;
     if (tid < nj) {
       const double* Tj = tab + static_cast<size_t>(tid) * L.TS;
       const bool noise = fabs(Tj[0]) <= 1e-12 * n_phi && fabs(Tj[1]) <= 1e-12 * n_phi;
@@ -321,25 +332,34 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
       if (item_s[tid] < 0 || rf_s[tid] == 0.0) lo = n_t;
       else if (drop_early) lo = tau0_s[tid] < kInf ? max(k_a, static_cast<int>(kap[tid * L.KS])) : n_t;
       tlo_s[tid] = lo;
+      col0 = cpi * (j0 + tid);  // the item's first column
     }
+    int lo_min = n_t;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off, 64));
-    if (tid == 0) {
-      m.ctlo[(static_cast<int64_t>(tile) * g.n_c + i) * 2 + ct] = lo;
-      lo_s = lo;
+    for (int c = 0; c < CTW; ++c) {
+      // items whose columns [col0, col0 + cpi) meet column tile ct0 + c
+      const int c16 = 16 * (ct0 + c);
+      int v = col0 < c16 + 16 && col0 + cpi > c16 ? lo : n_t;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+      if (tid == 0) m.ctlo[(static_cast<int64_t>(tile) * g.n_c + i) * 2 + ct0 + c] = v;
+      lo_min = min(lo_min, v);
     }
+    if (tid == 0) lo_s = lo_min;
   }
   __syncthreads();
-  // (4) this column tile's weight rows from the first one the products read:
-  // lane t writes the two columns (+, x) of one (item, set), 16 ct + 2 (t % 8)
-  // and the next, of rows 32 apart, sharing the collided count, the record and
-  // the time-table loads of a row between them
-  const int col = 16 * ct + 2 * (tid & 7);
+  // (4) the column tiles' weight rows from the first one the products read:
+  // lane t writes the two columns (+, x) of one (item, set), 16 ct0 + 2 (t % CP)
+  // and the next, of rows 256 / CP apart, sharing the collided count, the
+  // record and the time-table loads of a row between them
+  constexpr int CP = 8 * CTW;    // column pairs of the workgroup
+  constexpr int RG = 256 / CP;   // row groups
+  const int col = 16 * ct0 + 2 * (tid % CP);
   const int jg = col / cpi;           // the columns' item in the tile
   const int jc = jg - j0;             // and in this workgroup
   const int live0 = lo_s < n_t ? (lo_s & ~3) - info.kbase : info.len;
-  const int RP = (info.len - live0 + 31) / 32;
-  const int r0 = live0 + (tid >> 3) * RP;
+  const int RP = (info.len - live0 + RG - 1) / RG;
+  const int r0 = live0 + (tid / CP) * RP;
   const int r1 = min(info.len, r0 + RP);
   double* vout = m.vbuf + (info.voff - m.voff0) + static_cast<int64_t>(i) * info.len * ncol;
   const bool live = jg < ni && jc < nj && item_s[jc] >= 0;
@@ -603,13 +623,16 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
 
 hipError_t launch_weights(const MatArgs& a, int n_tiles, size_t lds, hipStream_t s) {
   const bool fast = a.s.n_sets == 1 && a.s.n_terms == 1;
-  auto k = fast ? mx_weights_kernel<true> : mx_weights_kernel<false>;
+  const int seg = mx_seg_lanes(a.s.n_phi, a.s.n_sets, a.s.n_terms, a.col_tiles);
+  auto k = !fast ? mx_weights_kernel<false, kMxSegLanes>
+                 : (seg == 16 ? mx_weights_kernel<true, 16> : mx_weights_kernel<true, kMxSegLanes>);
+  const int ctw = seg == 16 ? 2 : 1;
   std::lock_guard<std::mutex> lock(lds_limit_mutex());
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(n_tiles, a.s.n_c, a.col_tiles), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k, dim3(n_tiles, a.s.n_c, a.col_tiles / ctw), dim3(256), lds, s, a);
   return hipGetLastError();
 }
 
@@ -617,9 +640,11 @@ hipError_t launch_weights(const MatArgs& a, int n_tiles, size_t lds, hipStream_t
 
 int matrix_items_per_tile(int n_sets, int col_tiles) { return (16 * col_tiles) / (2 * n_sets); }
 
-size_t matrix_lds_bytes(const SpecArgs& a, int ni, int /*col_tiles*/) {
-  const MxLayout L = mx_layout(a.n_phi, a.n_sets, a.n_terms);
-  const size_t nj = static_cast<size_t>(ni < kMxWeightItems ? ni : kMxWeightItems);
+size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles) {
+  const int seg = mx_seg_lanes(a.n_phi, a.n_sets, a.n_terms, col_tiles);
+  const MxLayout L = mx_layout(a.n_phi, a.n_sets, a.n_terms, seg);
+  const int items = 256 / seg;
+  const size_t nj = static_cast<size_t>(ni < items ? ni : items);
   return nj * L.TS * sizeof(double) + nj * L.KS * sizeof(uint16_t);
 }
 
