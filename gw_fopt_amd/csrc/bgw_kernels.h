@@ -169,7 +169,7 @@ inline int resolve_threads(int threads, int n_w, int mode) {
   if (threads > 0) return threads;
   // omegas per lane by default, from a sweep of (block steps, lanes) at
   // plain_256 (256 omegas, phase-sum kernel per step): 2 for the blocked FP64
-  // form (8/128: 26.4 ms; 8/256: 29.2; 16/256: 33.9; 16/128: 38.2), 1
+  // form (8/128: 22.0 ms; 16/128: 22.1; 8/256: 28.6; 16/256: 33.2), 1
   // otherwise (mixed 16/256: 22.5 ms; 16/128: 23.5; 8/256: 24.9)
   const int per_lane = mode == 1 ? 2 : 1;
   int t = ((n_w + per_lane - 1) / per_lane + 63) / 64 * 64;

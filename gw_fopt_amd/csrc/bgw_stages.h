@@ -932,13 +932,36 @@ namespace {
 // ---------------------------------------------------------------------------
 // MODE 0: FP64, one double sincos per (t, omega) term (the parity form).
 // MODE 1: FP64, block-factorised phases: exp(i w psi(t0 + r)) = A(t0) E_r with
-//         E_r = exp(i w r dt (1 - cos th)); double E_r (recurrence), double
-//         block sums combined by Horner's rule in E_RB over a chunk's blocks,
-//         one exact anchor sincos per chunk.
-// MODE 2: mixed, the same factorisation and FP64 anchors, with the E_r table,
-//         weights and block sums in float; block results accumulated in float
-//         with Kahan compensation.
+//         E_r = exp(i w r dt (1 - cos th)), taken about each block's centre so
+//         that the steps pair up with conjugate phases E_{+-(j+1/2)} (the
+//         block's pair sums and differences formed once in LDS); double
+//         half-step table (recurrence), double block sums combined by Horner's
+//         rule in E_RB over a chunk's blocks, one exact anchor sincos per chunk.
+// MODE 2: mixed, the same factorisation and FP64 anchors, with the half-step
+//         table, weights, pair sums and block sums in float; block results
+//         accumulated in float with Kahan compensation.
 //
+// The block-centre pair sums of a chunk's weights, in place: for each block of
+// RB steps from l0 and j < RB / 2, the weights at steps l0 + RB/2 + j (a) and
+// l0 + RB/2 - 1 - j (b), +-(j + 1/2) steps from the block's centre, become
+// v_a + v_b (at b) and v_a - v_b (at a).  Each pair belongs to one lane; the
+// barrier after it publishes the sums.
+template <int RB, int NS, typename V>
+__device__ void pair_block_weights(V* v, int len) {
+  constexpr int RH = RB / 2;
+  const int n_pair = (len + RB - 1) / RB * RH * 2 * NS;
+  for (int u = threadIdx.x; u < n_pair; u += blockDim.x) {
+    const int c = u % (2 * NS), bj = u / (2 * NS);
+    const int l0 = bj / RH * RB, j = bj % RH;
+    V* pa = v + (l0 + RH + j) * 2 * NS + c;
+    V* pb = v + (l0 + RH - 1 - j) * 2 * NS + c;
+    const V va = *pa, vb = *pb;
+    *pb = va + vb;
+    *pa = va - vb;
+  }
+  __syncthreads();
+}
+
 // spectrum_item: one item's C contribution; lane l holds omegas l + q * blockDim.x,
 // res[q][s] = (Re C+, Im C+, Re Cx, Im Cx) of set s.  tau_item indexes g.tau /
 // g.perm.  Called by the whole workgroup.
@@ -984,22 +1007,28 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
       __syncthreads();
       continue;
     }
-    // E_r = exp(i omega r step (1 - cos th)), r <= RB, by a double recurrence;
-    // E_RB advances a block anchor to the next block.
-    constexpr int kE = MODE == 0 ? 1 : RB;
+    // The blocks of RB steps are anchored at their centre, (RB - 1) / 2 steps
+    // in: their steps lie at +-(j + 1/2) steps from it, j < RB / 2, with the
+    // half-step phases F_j = E_{j + 1/2} = E_j E_{1/2} and their conjugates,
+    // E_r = exp(i omega r step (1 - cos th)) by a double recurrence; E_RB
+    // advances a block anchor to the next block.
+    constexpr int kE = MODE == 0 ? 1 : RB / 2;
     Acc er[OPL][kE], ei[OPL][kE];
     double ebr[OPL], ebi[OPL];
+    const double delta = ri.t_step * (1.0 - g.ring_c[i]);
     if constexpr (MODE != 0) {
-      const double delta = ri.t_step * (1.0 - g.ring_c[i]);
 #pragma unroll
       for (int q = 0; q < OPL; ++q) {
-        double s1, c1;
+        double s1, c1, sh, ch;
         sincos_f64(om[q] * delta, &s1, &c1);
+        sincos_f64(om[q] * (0.5 * delta), &sh, &ch);
         double cr = 1.0, ci = 0.0;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
-          er[q][r] = static_cast<Acc>(cr);
-          ei[q][r] = static_cast<Acc>(ci);
+          if (r < kE) {
+            er[q][r] = static_cast<Acc>(cr * ch - ci * sh);
+            ei[q][r] = static_cast<Acc>(cr * sh + ci * ch);
+          }
           const double nr = cr * c1 - ci * s1;
           const double ni = cr * s1 + ci * c1;
           cr = nr;
@@ -1032,12 +1061,20 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           }
         }
       } else {
+        constexpr int RH = RB / 2;
         if constexpr (MODE == 1) {
           // FP64: Horner over the chunk's blocks, last to first,
-          //   H <- H E_RB + B_b,   C += exp(i w psi(t0)) H,
-          // with one exact FP64 sincos per chunk and omega.  A block's weights
-          // are loaded into registers before its FMA chain so the LDS reads
-          // overlap.
+          //   H <- H E_RB + B_b,   C += exp(i w psi(t0)) E_{(RB-1)/2} H,
+          // with one exact FP64 sincos per chunk and omega.  B_b is the block
+          // sum about the block's centre: its steps RB/2 + j and RB/2 - 1 - j
+          // sit at +-(j + 1/2) steps from it, with conjugate phases, so
+          //   v_a F_j + v_b conj(F_j) = (v_a + v_b) Re F_j + i (v_a - v_b) Im F_j
+          // and the sums and differences, formed once per chunk in place of the
+          // weights (the sum at step RB/2 - 1 - j, the difference at RB/2 + j),
+          // serve every lane's omegas: per block and omega, 2 RB multiply-adds
+          // per polarisation instead of 4 RB.  A block's sums are loaded into
+          // registers before its FMA chain so the LDS reads overlap.
+          pair_block_weights<RB, NS>(L.v, len);
           double hh[OPL][NS][4];
 #pragma unroll
           for (int q = 0; q < OPL; ++q)
@@ -1048,11 +1085,14 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           const int nb = (len + RB - 1) / RB;
           for (int b = nb - 1; b >= 0; --b) {
             const int l0 = b * RB;
-            double vv[RB][2 * NS];  // zero-padded past len
+            double sv[RH][2 * NS], dv[RH][2 * NS];  // weights zero-padded past len
 #pragma unroll
-            for (int r = 0; r < RB; ++r)
+            for (int j = 0; j < RH; ++j)
 #pragma unroll
-              for (int c = 0; c < 2 * NS; ++c) vv[r][c] = L.v[(l0 + r) * 2 * NS + c];
+              for (int c = 0; c < 2 * NS; ++c) {
+                sv[j][c] = L.v[(l0 + RH - 1 - j) * 2 * NS + c];
+                dv[j][c] = L.v[(l0 + RH + j) * 2 * NS + c];
+              }
 #pragma unroll
             for (int q = 0; q < OPL; ++q) {
 #pragma unroll
@@ -1062,18 +1102,12 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
                 double b1 = __builtin_fma(hh[q][s][0], ebi[q], hh[q][s][1] * ebr[q]);
                 double b2 = __builtin_fma(hh[q][s][2], ebr[q], -(hh[q][s][3] * ebi[q]));
                 double b3 = __builtin_fma(hh[q][s][2], ebi[q], hh[q][s][3] * ebr[q]);
-                // E_0 = 1 exactly: its term is one addition per polarisation
-                // (the same rounding as the multiply-add by 1; the product with
-                // Im E_0 = 0 would add a zero)
-                b0 += vv[0][2 * s];
-                b2 += vv[0][2 * s + 1];
 #pragma unroll
-                for (int r = 1; r < RB; ++r) {
-                  const double vp = vv[r][2 * s], vx = vv[r][2 * s + 1];
-                  b0 = __builtin_fma(vp, static_cast<double>(er[q][r]), b0);
-                  b1 = __builtin_fma(vp, static_cast<double>(ei[q][r]), b1);
-                  b2 = __builtin_fma(vx, static_cast<double>(er[q][r]), b2);
-                  b3 = __builtin_fma(vx, static_cast<double>(ei[q][r]), b3);
+                for (int j = 0; j < RH; ++j) {
+                  b0 = __builtin_fma(sv[j][2 * s], static_cast<double>(er[q][j]), b0);
+                  b1 = __builtin_fma(dv[j][2 * s], static_cast<double>(ei[q][j]), b1);
+                  b2 = __builtin_fma(sv[j][2 * s + 1], static_cast<double>(er[q][j]), b2);
+                  b3 = __builtin_fma(dv[j][2 * s + 1], static_cast<double>(ei[q][j]), b3);
                 }
                 hh[q][s][0] = b0;
                 hh[q][s][1] = b1;
@@ -1085,8 +1119,13 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           const double psi0 = L.psi[0];
 #pragma unroll
           for (int q = 0; q < OPL; ++q) {
-            double as, ac;
-            sincos_f64(om[q] * psi0, &as, &ac);
+            // the anchor of the first block's centre: exp(i w psi(t0)) F_{RH-1}
+            double as0, ac0;
+            sincos_f64(om[q] * psi0, &as0, &ac0);
+            const double fr = static_cast<double>(er[q][RH - 1]);
+            const double fi = static_cast<double>(ei[q][RH - 1]);
+            const double ac = __builtin_fma(ac0, fr, -(as0 * fi));
+            const double as = __builtin_fma(ac0, fi, as0 * fr);
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
               acc[q][s][0] += __builtin_fma(ac, hh[q][s][0], -(as * hh[q][s][1]));
@@ -1097,44 +1136,48 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
           }
         } else {
         // Mixed: groups of kReanchor blocks; an exact FP64 sincos gives the
-        // anchor exp(i w psi) of the group's first block, each following
-        // block's anchor is the previous one times E_RB (FP64); block sums in
-        // float, rotated by the anchor and added with Kahan compensation.
+        // anchor exp(i w psi) of the group's first block centre, each following
+        // block's anchor is the previous one times E_RB (FP64); block sums
+        // about the centre, from the float pair sums and differences as in
+        // the FP64 form, rotated by the anchor and added with Kahan
+        // compensation.
+        pair_block_weights<RB, NS>(L.vf, len);
+        const double half = 0.5 * static_cast<double>(RB - 1) * delta;
         for (int g0 = 0; g0 < len; g0 += kReanchor * RB) {
           double anc_r[OPL], anc_i[OPL];
           const double psi0 = L.psi[g0];
 #pragma unroll
-          for (int q = 0; q < OPL; ++q) sincos_f64(om[q] * psi0, &anc_i[q], &anc_r[q]);
+          for (int q = 0; q < OPL; ++q) sincos_f64(om[q] * (psi0 + half), &anc_i[q], &anc_r[q]);
           const int g1 = min(len, g0 + kReanchor * RB);
           for (int l0 = g0; l0 < g1; l0 += RB) {
-            Acc vv[RB][2 * NS];  // zero-padded past len
+            Acc sv[RH][2 * NS], dv[RH][2 * NS];  // weights zero-padded past len
 #pragma unroll
-            for (int r = 0; r < RB; ++r)
+            for (int j = 0; j < RH; ++j)
 #pragma unroll
-              for (int c = 0; c < 2 * NS; ++c) vv[r][c] = L.vf[(l0 + r) * 2 * NS + c];
-            // the block sums start from the E_0 = 1 term
+              for (int c = 0; c < 2 * NS; ++c) {
+                sv[j][c] = L.vf[(l0 + RH - 1 - j) * 2 * NS + c];
+                dv[j][c] = L.vf[(l0 + RH + j) * 2 * NS + c];
+              }
             Acc bs[OPL][NS][4];
 #pragma unroll
             for (int q = 0; q < OPL; ++q)
 #pragma unroll
               for (int s = 0; s < NS; ++s) {
-                bs[q][s][0] = vv[0][2 * s];
-                bs[q][s][1] = Acc(0);
-                bs[q][s][2] = vv[0][2 * s + 1];
-                bs[q][s][3] = Acc(0);
+                bs[q][s][0] = sv[0][2 * s] * er[q][0];
+                bs[q][s][1] = dv[0][2 * s] * ei[q][0];
+                bs[q][s][2] = sv[0][2 * s + 1] * er[q][0];
+                bs[q][s][3] = dv[0][2 * s + 1] * ei[q][0];
               }
 #pragma unroll
-            for (int r = 1; r < RB; ++r) {
+            for (int j = 1; j < RH; ++j) {
 #pragma unroll
               for (int s = 0; s < NS; ++s) {
-                const Acc vp = vv[r][2 * s];
-                const Acc vx = vv[r][2 * s + 1];
 #pragma unroll
                 for (int q = 0; q < OPL; ++q) {
-                  bs[q][s][0] = fma(vp, er[q][r], bs[q][s][0]);
-                  bs[q][s][1] = fma(vp, ei[q][r], bs[q][s][1]);
-                  bs[q][s][2] = fma(vx, er[q][r], bs[q][s][2]);
-                  bs[q][s][3] = fma(vx, ei[q][r], bs[q][s][3]);
+                  bs[q][s][0] = fma(sv[j][2 * s], er[q][j], bs[q][s][0]);
+                  bs[q][s][1] = fma(dv[j][2 * s], ei[q][j], bs[q][s][1]);
+                  bs[q][s][2] = fma(sv[j][2 * s + 1], er[q][j], bs[q][s][2]);
+                  bs[q][s][3] = fma(dv[j][2 * s + 1], ei[q][j], bs[q][s][3]);
                 }
               }
             }
