@@ -98,12 +98,4 @@ __device__ __forceinline__ void sincos_f64(double x, double* sin_out, double* co
   apply_quadrant(n, s, c, sin_out, cos_out);
 }
 
-// Kahan-compensated float accumulation: sum += x.
-__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
-  float y = x - comp;
-  float t = sum + y;
-  comp = (t - sum) - y;
-  sum = t;
-}
-
 }  // namespace bgw

@@ -163,14 +163,14 @@ hipError_t launch_matrix_phases(const MatArgs& a, int n_real, hipStream_t s);
 // 16 otherwise; workgroups of 64..256 lanes holding one omega each in the
 // FP64 form and two in the blocked forms.
 inline int resolve_block_steps(int block_steps, int mode) {
-  return block_steps ? block_steps : (mode == 1 ? 8 : 16);
+  return block_steps ? block_steps : 8;
 }
 inline int resolve_threads(int threads, int n_w, int mode) {
   if (threads > 0) return threads;
   // omegas per lane by default, from a sweep of (block steps, lanes) at
   // plain_256 (256 omegas, phase-sum kernel per step): 2 for the blocked FP64
   // form (8/128: 22.0 ms; 16/128: 22.1; 8/256: 28.6; 16/256: 33.2), 1
-  // otherwise (mixed 16/256: 22.5 ms; 16/128: 23.5; 8/256: 24.9)
+  // otherwise (mixed 8/256: 18.7 ms; 16/256: 19.8)
   const int per_lane = mode == 1 ? 2 : 1;
   int t = ((n_w + per_lane - 1) / per_lane + 63) / 64 * 64;
   return t < 64 ? 64 : (t > 256 ? 256 : t);

@@ -27,13 +27,15 @@
 //   psi = (t - z_a) - (t - t_a) cos th, in registers:
 //     FP64 form:  one double sincos per (t, omega) term;
 //     blocked FP64 form: the phase of a block of R steps is anchor * E_r in
-//                 double, blocks combined by Horner's rule, one exact anchor
-//                 per chunk of steps;
+//                 double, about the block's centre (conjugate step pairs
+//                 share a phase), blocks combined by Horner's rule, one exact
+//                 anchor per chunk of steps;
 //     mixed form: psi arguments reduced in double; the phase of a block of
 //                 R consecutive steps factorised as anchor * E_r with
-//                 E_r = exp(i omega r dt (1 - cos th)); exact double anchors
-//                 every 16 blocks, float phase factors E_r, weights and block
-//                 sums, Kahan-compensated float accumulation.
+//                 E_r = exp(i omega r dt (1 - cos th)), about the block's
+//                 centre; one exact double anchor per chunk advanced by E_R in
+//                 double, float phase factors, weights and block sums, block
+//                 results rotated and accumulated in double.
 //
 // Stage 3 (reduce_kernel): sum the per-bubble partial C in bubble order
 //   (deterministic) and form S = 8 pi w^3 (|C+|^2 + |Cx|^2).

@@ -671,15 +671,13 @@ __device__ __forceinline__ double cutoff_weight(const SpecArgs& g, double t) {
 }
 
 // Time steps per chunk of per-step weights in LDS (SpecArgs::time_chunk): one
-// exact anchor per chunk and omega in the blocked form.  Both lengths are
-// multiples of the mixed form's anchor groups (16 blocks of 16 steps).
+// exact anchor per chunk and omega in the blocked and mixed forms.  Both
+// lengths are multiples of every block length (8, 16).
 constexpr int kTimeChunkMin = 256;
 constexpr int kTimeChunkMax = 512;
 __host__ __device__ inline int time_chunk_of(const SpecArgs& a) {
   return a.time_chunk == kTimeChunkMax ? kTimeChunkMax : kTimeChunkMin;
 }
-// Blocked forms: an exact anchor sincos every kReanchor blocks of a chunk.
-constexpr int kReanchor = 16;
 
 // Per-ring tables in LDS, laid out after each other, then the per-item time
 // tables (when they fit) and the per-chunk phase inputs.
@@ -939,7 +937,7 @@ namespace {
 //         rule in E_RB over a chunk's blocks, one exact anchor sincos per chunk.
 // MODE 2: mixed, the same factorisation and FP64 anchors, with the half-step
 //         table, weights, pair sums and block sums in float; block results
-//         accumulated in float with Kahan compensation.
+//         rotated by the FP64 anchors and accumulated in FP64.
 //
 // The block-centre pair sums of a chunk's weights, in place: for each block of
 // RB steps from l0 and j < RB / 2, the weights at steps l0 + RB/2 + j (a) and
@@ -983,21 +981,14 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
     const int wi = threadIdx.x + q * blockDim.x;
     om[q] = wi < g.n_w ? g.omega[ri.w_off + wi] : 0.0;
   }
-  // running sums (double), or float sums + Kahan compensations in MODE 2
-  Acc acc[OPL][NS][4];
-  float comp[MODE == 2 ? OPL : 1][MODE == 2 ? NS : 1][4];
+  // running sums, double in every form
+  double acc[OPL][NS][4];
 #pragma unroll
   for (int q = 0; q < OPL; ++q)
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[q][s][c] = Acc(0);
-#pragma unroll
-  for (int q = 0; q < (MODE == 2 ? OPL : 1); ++q)
-#pragma unroll
-    for (int s = 0; s < (MODE == 2 ? NS : 1); ++s)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) comp[q][s][c] = 0.0f;
+      for (int c = 0; c < 4; ++c) acc[q][s][c] = 0.0;
 
   for (int i = 0; i < g.n_c; ++i) {
     if (g.ring_fac[static_cast<int64_t>(surf) * g.n_c + i] == 0.0) continue;  // B = 0 exactly
@@ -1019,9 +1010,18 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
     if constexpr (MODE != 0) {
 #pragma unroll
       for (int q = 0; q < OPL; ++q) {
-        double s1, c1, sh, ch;
-        sincos_f64(om[q] * delta, &s1, &c1);
+        // E_{1/2} by a sincos; E_1 by its own in the FP64 form (its square
+        // doubles the rounding of E_1, which the Horner steps carry through
+        // the chunk), as the square of E_{1/2} in the mixed form (whose float
+        // tables round far above it)
+        double sh, ch, s1, c1;
         sincos_f64(om[q] * (0.5 * delta), &sh, &ch);
+        if constexpr (MODE == 1) {
+          sincos_f64(om[q] * delta, &s1, &c1);
+        } else {
+          c1 = __builtin_fma(ch, ch, -(sh * sh));
+          s1 = 2.0 * (sh * ch);
+        }
         double cr = 1.0, ci = 0.0;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
@@ -1135,21 +1135,19 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
             }
           }
         } else {
-        // Mixed: groups of kReanchor blocks; an exact FP64 sincos gives the
-        // anchor exp(i w psi) of the group's first block centre, each following
-        // block's anchor is the previous one times E_RB (FP64); block sums
-        // about the centre, from the float pair sums and differences as in
-        // the FP64 form, rotated by the anchor and added with Kahan
-        // compensation.
+        // Mixed: an exact FP64 sincos gives the anchor exp(i w psi) of the
+        // chunk's first block centre, each following block's anchor is the
+        // previous one times E_RB (FP64, as the FP64 form's Horner steps);
+        // block sums about the centre, from the float pair sums and
+        // differences as in the FP64 form, rotated by the anchor and
+        // accumulated in FP64.
         pair_block_weights<RB, NS>(L.vf, len);
-        const double half = 0.5 * static_cast<double>(RB - 1) * delta;
-        for (int g0 = 0; g0 < len; g0 += kReanchor * RB) {
+        {
           double anc_r[OPL], anc_i[OPL];
-          const double psi0 = L.psi[g0];
+          const double psi0 = L.psi[0] + 0.5 * static_cast<double>(RB - 1) * delta;
 #pragma unroll
-          for (int q = 0; q < OPL; ++q) sincos_f64(om[q] * (psi0 + half), &anc_i[q], &anc_r[q]);
-          const int g1 = min(len, g0 + kReanchor * RB);
-          for (int l0 = g0; l0 < g1; l0 += RB) {
+          for (int q = 0; q < OPL; ++q) sincos_f64(om[q] * psi0, &anc_i[q], &anc_r[q]);
+          for (int l0 = 0; l0 < len; l0 += RB) {
             Acc sv[RH][2 * NS], dv[RH][2 * NS];  // weights zero-padded past len
 #pragma unroll
             for (int j = 0; j < RH; ++j)
@@ -1183,19 +1181,19 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
             }
 #pragma unroll
             for (int q = 0; q < OPL; ++q) {
-              const Acc as = static_cast<Acc>(anc_i[q]);
-              const Acc ac = static_cast<Acc>(anc_r[q]);
+              const double as = anc_i[q], ac = anc_r[q];
 #pragma unroll
               for (int s = 0; s < NS; ++s) {
-                // (ac + i as) * (b0 + i b1), likewise for the cross polarisation
-                const Acc p0 = fma(ac, bs[q][s][0], -as * bs[q][s][1]);
-                const Acc p1 = fma(ac, bs[q][s][1], as * bs[q][s][0]);
-                const Acc p2 = fma(ac, bs[q][s][2], -as * bs[q][s][3]);
-                const Acc p3 = fma(ac, bs[q][s][3], as * bs[q][s][2]);
-                kahan_add(acc[q][s][0], comp[q][s][0], p0);
-                kahan_add(acc[q][s][1], comp[q][s][1], p1);
-                kahan_add(acc[q][s][2], comp[q][s][2], p2);
-                kahan_add(acc[q][s][3], comp[q][s][3], p3);
+                // C += (ac + i as) (b0 + i b1), likewise for the cross
+                // polarisation: the float block sums widened, rotated by the
+                // FP64 anchor and accumulated in FP64 (two multiply-adds per
+                // component, at the rate of the float ones)
+                const double b0 = bs[q][s][0], b1 = bs[q][s][1];
+                const double b2 = bs[q][s][2], b3 = bs[q][s][3];
+                acc[q][s][0] = __builtin_fma(-as, b1, __builtin_fma(ac, b0, acc[q][s][0]));
+                acc[q][s][1] = __builtin_fma(as, b0, __builtin_fma(ac, b1, acc[q][s][1]));
+                acc[q][s][2] = __builtin_fma(-as, b3, __builtin_fma(ac, b2, acc[q][s][2]));
+                acc[q][s][3] = __builtin_fma(as, b2, __builtin_fma(ac, b3, acc[q][s][3]));
               }
               const double nr = __builtin_fma(anc_r[q], ebr[q], -(anc_i[q] * ebi[q]));
               const double ni = __builtin_fma(anc_r[q], ebi[q], anc_i[q] * ebr[q]);
@@ -1214,13 +1212,7 @@ __device__ void spectrum_item(const SpecArgs& g, int item_id, int64_t tau_item, 
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if constexpr (MODE == 2) {
-          res[q][s][j] = static_cast<double>(acc[q][s][j]) - static_cast<double>(comp[q][s][j]);
-        } else {
-          res[q][s][j] = static_cast<double>(acc[q][s][j]);
-        }
-      }
+      for (int j = 0; j < 4; ++j) res[q][s][j] = acc[q][s][j];
 }
 
 }  // namespace

@@ -68,7 +68,7 @@ typedef struct bgw_generalized_bulk_flow bgw_generalized_bulk_flow;
  *  Mixed:       collision tests and phase-argument reduction in double; the
  *               phases of R consecutive time steps factorised as anchor x
  *               table, sincos, tables, weights and block sums in float, block
- *               results accumulated in float with Kahan compensation.
+ *               results rotated and accumulated in double.
  *  Fp64Blocked: the same block factorisation entirely in double.
  *  Fp64Matrix:  double throughout, on the matrix cores: the phase sums of the
  *               items of one bubble in several k-hat frames (same t_a) are one

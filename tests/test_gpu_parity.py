@@ -249,6 +249,30 @@ def test_spectrum_multi_chunk_blocked_and_mixed():
     assert np.median(rel) <= MIXED_REL
 
 
+@pytest.mark.parametrize("block_steps", [8, 16])
+@pytest.mark.parametrize("n_t", [13, 37, 515])
+def test_blocked_and_mixed_block_lengths(block_steps, n_t):
+    """Both block lengths of the factorised forms, whose block sums pair the steps about
+    each block's centre, on step counts that end in partial blocks (shorter than half a
+    block included) and, at 515, in a partial chunk: the blocked form against the oracle,
+    the mixed form against FP64."""
+    wl = configs.plain_8()
+    k = wl.k_hat[:4]
+    bf = flow(wl.interior, wl.exterior, wl.n_cos_thetax, wl.n_phix, wl.coefficients_sets,
+              wl.powers_sets)
+    So = oracle.spectrum(wl.interior, wl.exterior, k, wl.n_cos_thetax, wl.n_phix,
+                         wl.coefficients_sets, wl.powers_sets, wl.w_arr, t_end=wl.t_end, n_t=n_t)
+    S64, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=n_t)
+    assert rel_per_bin(S64, So) <= FP64_REL
+    bf.set_tiles(block_steps, 0)
+    Sb, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=n_t, precision="fp64_blocked")
+    assert rel_per_bin(Sb, So) <= FP64_REL
+    S32, _ = bf.compute_spectrum(wl.w_arr, k, t_end=wl.t_end, n_t=n_t, precision="mixed")
+    rel = np.abs(S32 - S64) / np.abs(S64)
+    assert rel.max() <= 1e-4
+    assert np.median(rel) <= MIXED_REL
+
+
 def test_identity_frame_equals_reference_frame():
     """k-hat = z-hat reproduces the reference-frame call bit for bit."""
     wl = configs.plain_8()
