@@ -806,7 +806,7 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
           if ((ri.t_begin + ri.t_step * static_cast<double>(mid)) - t_a > 0.0) hi = mid;
           else lo = mid + 1;
         }
-        kbase_of.push_back(lo & ~3);
+        kbase_of.push_back(lo & ~(bgw::kMatrixBlockSteps - 1));
         k_a_of.push_back(lo);
       }
   }
@@ -825,7 +825,8 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
                    tiles.begin() + static_cast<size_t>(c + 1) * ma.ni);
     info[t].kbase = kbase_of[c];
     info[t].k_a = k_a_of[c];
-    info[t].len = (static_cast<int32_t>(q.n_t) - kbase_of[c] + 3) & ~3;
+    info[t].len = (static_cast<int32_t>(q.n_t) - kbase_of[c] + bgw::kMatrixBlockSteps - 1) &
+                  ~(bgw::kMatrixBlockSteps - 1);
     info[t].voff = voff;
     voff += static_cast<int64_t>(sb.n_c) * info[t].len * static_cast<int64_t>(ncol);
   }
@@ -838,7 +839,7 @@ MatrixPlan plan_matrix_form(Engine& E, const Request& q, const Launch& Lc,
   ma.tinfo = E.d_tinfo.as<bgw::MxTileInfo>();
   ma.tt = E.d_tt.as<double>();
   ma.ctlo = E.d_ctlo.as<int32_t>();
-  E.d_mxphase.ensure(std::max<size_t>(1, n_real * sb.n_c * q.n_w * 4) * sizeof(double));
+  E.d_mxphase.ensure(std::max<size_t>(1, n_real * sb.n_c * q.n_w * 8) * sizeof(double));
   ma.phase = E.d_mxphase.as<double>();
   // batches of whole tiles within the row budget (a tile larger than the budget alone)
   std::vector<std::pair<size_t, size_t>> batches;

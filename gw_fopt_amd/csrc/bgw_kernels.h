@@ -125,7 +125,7 @@ hipError_t launch_ensemble_stats(const double* s_spec, int n_real, int n_k, int 
 // ring, at tinfo[tile].voff - voff0), and the matrix products into partial.
 struct MxTileInfo {
   int64_t voff;           // first double of the tile's weight rows
-  int32_t kbase;          // step of row 0: the first weighted step of the bubble, rounded down to 4
+  int32_t kbase;          // step of row 0: the first weighted step of the bubble, rounded down to 16
   int32_t len;            // rows per ring: n_t - kbase rounded up to 4
   int32_t k_a;            // the first step with t_k - t_a > 0 (n_t if none)
 };
@@ -148,8 +148,11 @@ struct MatArgs {
 // n_sets * n_terms of one matrix-form launch
 constexpr int kMaxMatrixTerms = 12;
 // rows of padding (of 32 doubles) after the weight rows of a launch: the
-// products kernel prefetches up to 28 rows past a ring's last row
+// products kernel prefetches up to one block of rows past a ring's last row
 constexpr int kMatrixRowPad = 32;
+// time steps per block of the matrix form's products: a tile's first weight
+// row (kbase) and its row count per ring are multiples of it
+constexpr int kMatrixBlockSteps = 16;
 int matrix_items_per_tile(int n_sets, int col_tiles);
 size_t matrix_lds_bytes(const SpecArgs& a, int ni, int col_tiles);
 // the time tables and weight rows of the launch's tiles, then their matrix products

@@ -30,18 +30,23 @@
 //    arithmetic of the other forms' weights
 //    (compute_b_integral, :481-586, folded with the quadrature weights of
 //    :588-664, 775-873).  Latency-bound work, run at full occupancy.
-//  * mx_phase_kernel, once per call: per realisation, ring and omega the step
-//    and anchor factors below (the same values for every tile).
-//  * mx_gemm_kernel, one workgroup per (tile, block of omegas): each wavefront
-//    streams the weight rows of every ring as the B operand (loaded four
-//    matrix steps ahead, from a pointer with immediate offsets) and generates
-//    the A operand, one (omega, step) element per lane advanced four steps at
-//    a time by an FP64 complex product with exp(i w 4 dt (1 - c_i)), restarted
-//    every 64 steps from an anchor that itself advances by exp(i w 64 dt (1 -
-//    c_i)) from the ring's exact FP64 phase; no LDS and no barriers.  FP64
+//  * mx_phase_kernel, once per call: per realisation, ring and omega the
+//    phase factors below (the same values for every tile).
+//  * mx_gemm_kernel, one workgroup per (tile, block of omegas): the steps of
+//    a ring go in blocks of 16 on the tile's grid; about a block's centre its
+//    steps pair up as 7 - j and 8 + j, j < 8, with conjugate phases
+//    F_j = exp(i w (j + 1/2) dt (1 - c_i)), so the block sum is
+//      B(w, col) = sum_j (V_a + V_b) Re F_j + i (V_a - V_b) Im F_j:
+//    two K = 4 slices per part, the A operands Re F_j, Im F_j fixed per ring
+//    and omega, the B operands the sums and differences of the two weight
+//    rows each lane loads (one block ahead, from a pointer with immediate
+//    offsets) -- half the matrix instructions of the direct phase sum.  The
+//    blocks accumulate by Horner's rule, H <- H conj(E_16) + B_b (four FP64
+//    products per accumulator element and block), and the rings through one
+//    complex factor per omega: the running sum is A_last H, A_last the exact
+//    FP64 phase of the last block centre; no LDS and no barriers.  FP64
 //    matrix instructions share the vector unit's FP64 datapath (they never
-//    co-execute), so every vector instruction here costs matrix time: the
-//    loop keeps them to the phase products.
+//    co-execute), so every vector instruction here costs matrix time.
 // Operand layouts of v_mfma_f64_16x16x4_f64 (measured on the device): A[m][k]
 // in lane m + 16 k, B[k][n] in lane n + 16 k, D[m][n] in element m / 4 of lane
 // n + 16 (m % 4).  The accumulators stay in VGPRs (this file is compiled with
@@ -57,11 +62,11 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// matrix steps (of 4 time steps) from one anchor of the phase factors to the
-// next: at most 15 step products in between; the anchors themselves advance
-// by one complex product each from the ring's exact FP64 phase at its first
-// step (a ring of n_t steps carries at most n_t / 64 of them)
-constexpr int kMxAnchorSteps = 16;
+// time steps per block of the products: pairs (B/2 - 1 - j, B/2 + j), j < B/2,
+// about the block's centre; the tiles' first rows (kbase) and row counts are
+// multiples of it
+constexpr int kMxBlock = kMatrixBlockSteps;
+static_assert(kMxBlock % 8 == 0 && kMxBlock <= kMatrixRowPad, "blocks of whole K slices, prefetched into the padding");
 
 // number of entries of the ascending row[0..n) that are <= k
 __device__ __forceinline__ int count_steps_le(const uint16_t* row, int n, int k) {
@@ -106,7 +111,6 @@ __device__ void wave_scan_strided(double* arr, int n, int stride, bool reverse) 
 // one double of padding per lane's segment, so the lanes of an item touch
 // different bank pairs when they walk their segments together.
 constexpr int kMxSegLanes = 32;
-constexpr int kMxWeightItems = 256 / kMxSegLanes;  // items of one weights workgroup
 
 // Lanes per item of the weights workgroups: 32, or 16 for one set of one term
 // in two column tiles of rings of at most 64 directions -- then a workgroup
@@ -173,9 +177,10 @@ __global__ __launch_bounds__(256) void mx_time_kernel(MatArgs m) {
 //    records.
 //  * rows: lane t writes the two columns (+, x) 16 ct0 + 2 (t % CP) and the
 //    next (CP = 8 CTW column pairs) of rows from the workgroup's first
-//    weighted step (rounded down to 4; the products read nothing of a column
-//    tile before its own), 256 / CP rows apart; the collided count is found
-//    once by a binary search and carried along the lane's rows.
+//    weighted step (rounded down to its block of 16 steps; the products read
+//    nothing of a column tile before its own block), 256 / CP rows apart; the
+//    collided count is found once by a binary search and carried along the
+//    lane's rows.
 // FAST: one set of one term.
 template <bool FAST, int SEG>
 __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
@@ -322,7 +327,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   // (the tile's first live step k_a comes with its tile info), by the first
   // wavefront, which also takes the minimum over each column tile's items
   if (tid < 64) {
-    int lo = n_t, col0 = -1 << 20;
+    int lo = n_t, col0 = -(1 << 20);
     if (tid < nj) {
       const double* Tj = tab + static_cast<size_t>(tid) * L.TS;
       const bool noise = fabs(Tj[0]) <= 1e-12 * n_phi && fabs(Tj[1]) <= 1e-12 * n_phi;
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   const int col = 16 * ct0 + 2 * (tid % CP);
   const int jg = col / cpi;           // the columns' item in the tile
   const int jc = jg - j0;             // and in this workgroup
-  const int live0 = lo_s < n_t ? (lo_s & ~3) - info.kbase : info.len;
+  const int live0 = lo_s < n_t ? (lo_s & ~(kMxBlock - 1)) - info.kbase : info.len;
   const int RP = (info.len - live0 + RG - 1) / RG;
   const int r0 = live0 + (tid / CP) * RP;
   const int r1 = min(info.len, r0 + RP);
@@ -413,9 +418,11 @@ __global__ __launch_bounds__(256) void mx_weights_kernel(MatArgs m) {
   }
 }
 
-// Per (realisation, ring, omega): the step factor e^{i w 4 dt (1 - c_i)} and
-// the anchor factor e^{i w 4 kMxAnchorSteps dt (1 - c_i)} of the products
-// kernel, once per call instead of once per tile and ring.
+// Per (realisation, ring, omega), with d = dt (1 - c_i): E_B = e^{i w B d}
+// (B = kMxBlock),
+// E_{1/2} = e^{i w d / 2}, E_1 = e^{i w d} and E_4 = e^{i w 4 d}, the phase
+// factors of the products kernel, once per call instead of once per tile and
+// ring.
 __global__ __launch_bounds__(256) void mx_phase_kernel(MatArgs m, int n_real) {
   const SpecArgs& g = m.s;
   const int64_t n = static_cast<int64_t>(n_real) * g.n_c * g.n_w;
@@ -427,19 +434,26 @@ __global__ __launch_bounds__(256) void mx_phase_kernel(MatArgs m, int n_real) {
   const RealInfo ri = g.real[r];
   const double om = g.omega[ri.w_off + wi];
   const double ai = 1.0 - g.ring_c[i];
-  double* p = m.phase + e * 4;
-  double sr, si, fr, fi;
-  sincos_f64(om * (4.0 * ri.t_step * ai), &si, &sr);
-  sincos_f64(om * ((4.0 * kMxAnchorSteps) * ri.t_step * ai), &fi, &fr);
-  p[0] = sr;
-  p[1] = si;
-  p[2] = fr;
-  p[3] = fi;
+  const double d = ri.t_step * ai;
+  double* p = m.phase + e * 8;
+  double sn, cs;
+  sincos_f64(om * (static_cast<double>(kMxBlock) * d), &sn, &cs);
+  p[0] = cs;
+  p[1] = sn;
+  sincos_f64(om * (0.5 * d), &sn, &cs);
+  p[2] = cs;
+  p[3] = sn;
+  sincos_f64(om * d, &sn, &cs);
+  p[4] = cs;
+  p[5] = sn;
+  sincos_f64(om * (4.0 * d), &sn, &cs);
+  p[6] = cs;
+  p[7] = sn;
 }
 
 // One (tile, block of omegas): wavefront w takes omega tiles w OT + [0, OT) of
-// the block and every column tile; the weight rows stream from HBM four
-// matrix steps ahead.  ct_lo: first weighted step of each column tile.
+// the block and every column tile; the weight rows stream from HBM one block
+// of steps ahead.  ct_lo: first weighted step of each column tile.
 template <int OT, int CT>
 __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
   constexpr int NCOL = 16 * CT;
@@ -488,7 +502,29 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
       acc_re[o][c] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_im[o][c] = v4d{0.0, 0.0, 0.0, 0.0};
     }
+  // the accumulators hold, per (omega, column), H with the running sum of the
+  // processed rings equal to A_last H, A_last = exp(i w psi) at the centre of
+  // the last processed block; lr/li: A_last for this lane's omega
+  double lr[OT], li[OT];
+#pragma unroll
+  for (int o = 0; o < OT; ++o) {
+    lr[o] = 0.0;
+    li[o] = 0.0;
+  }
+  bool any_ring = false;
+  // the element rows of this lane's accumulators: omega row kk + 4 r of each
+  // omega tile, held (as the A operand's row) by lane kk + 4 r
   const double* vt = m.vbuf + (info.voff - m.voff0) + (lane & 15);
+  // this lane's rows of a block (kMxBlock steps from its start), for the
+  // pairs j = 4 sl + kk about the block's centre: a = H + j, b = H - 1 - j,
+  // H = kMxBlock / 2
+  constexpr int NSL = kMxBlock / 8;  // K slices of four pairs
+  int off_a[NSL], off_b[NSL];
+#pragma unroll
+  for (int sl = 0; sl < NSL; ++sl) {
+    off_a[sl] = (kMxBlock / 2 + 4 * sl + kk) * NCOL;
+    off_b[sl] = (kMxBlock / 2 - 1 - 4 * sl - kk) * NCOL;
+  }
   for (int i = 0; i < g.n_c; ++i) {
     const int64_t ti = static_cast<int64_t>(tile) * g.n_c + i;
     int ct_lo[CT];
@@ -499,95 +535,154 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
       ks = min(ks, ct_lo[c]);
     }
     if (ks >= n_t) continue;
-    const int k0 = ks & ~3;
+    // blocks of kMxBlock steps on the tile's grid (kbase is a multiple of it)
+    const int k0 = ks & ~(kMxBlock - 1);
+    const int nblk = (n_t - k0 + kMxBlock - 1) / kMxBlock;
     const double ci = g.ring_c[i];
     const double ai = 1.0 - ci;
-    // step factor e^{i w 4 dt (1 - c)}, anchor factor e^{i w 4 kMxAnchorSteps dt (1 - c)},
-    // and the exact phase of matrix step 0
-    double sr[OT], si[OT], fr[OT], fi[OT], er[OT], ei[OT];
-    const double tk0 = ri.t_begin + ri.t_step * static_cast<double>(k0 + kk);
-    const double* ph = m.phase + (static_cast<int64_t>(it0.r) * g.n_c + i) * g.n_w * 4;
+    const double* ph = m.phase + (static_cast<int64_t>(it0.r) * g.n_c + i) * g.n_w * 8;
+    // A operands: F_j = exp(i w (j + 1/2) dt (1 - c)) for j = 4 sl + kk at
+    // this lane's omega; E_B = exp(i w kMxBlock dt (1 - c)) (the block's
+    // factor); the exact phases of the first and the last block centres
+    double far[OT][NSL], fai[OT][NSL], ebr[OT], ebi[OT], a0r[OT], a0i[OT], alr[OT], ali[OT];
+    constexpr double kCentre = 0.5 * (kMxBlock - 1);
+    const double tc0 = ri.t_begin + ri.t_step * (static_cast<double>(k0) + kCentre);
+    const double tcl =
+        ri.t_begin + ri.t_step * (static_cast<double>(k0 + kMxBlock * (nblk - 1)) + kCentre);
 #pragma unroll
     for (int o = 0; o < OT; ++o) {
       const int wi = wbase + o * 16 + (lane & 15);
-      sr[o] = 1.0;
-      si[o] = 0.0;
-      fr[o] = 1.0;
-      fi[o] = 0.0;
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl) far[o][sl] = fai[o][sl] = 0.0;
+      ebr[o] = 1.0;
+      ebi[o] = 0.0;
       if (wi < g.n_w) {
-        const double* p = ph + static_cast<int64_t>(wi) * 4;
-        sr[o] = p[0];
-        si[o] = p[1];
-        fr[o] = p[2];
-        fi[o] = p[3];
+        const double* p = ph + static_cast<int64_t>(wi) * 8;
+        ebr[o] = p[0];
+        ebi[o] = p[1];
+        double fr = p[2], fi = p[3];  // E_{1/2}
+        const double e1r = p[4], e1i = p[5];
+        for (int u = 0; u < kk; ++u) {
+          const double nr = __builtin_fma(fr, e1r, -(fi * e1i));
+          const double nim = __builtin_fma(fr, e1i, fi * e1r);
+          fr = nr;
+          fi = nim;
+        }
+#pragma unroll
+        for (int sl = 0; sl < NSL; ++sl) {
+          far[o][sl] = fr;
+          fai[o][sl] = fi;
+          const double nr = __builtin_fma(fr, p[6], -(fi * p[7]));  // times E_4
+          const double nim = __builtin_fma(fr, p[7], fi * p[6]);
+          fr = nr;
+          fi = nim;
+        }
       }
-      sincos_f64(om[o] * (tk0 * ai + t_a * ci), &ei[o], &er[o]);
+      sincos_f64(om[o] * (tc0 * ai + t_a * ci), &a0i[o], &a0r[o]);
+      sincos_f64(om[o] * (tcl * ai + t_a * ci), &ali[o], &alr[o]);
     }
-    double xr[OT], xi[OT];  // the anchor of the current group of kMxAnchorSteps steps
+    // the Horner factor conj(E_B) at the element rows, and the carry of the
+    // processed rings into this ring's H: H <- H A_last,prev conj(A_0) E_B
+    // (the ring's blocks then multiply it by conj(E_B)^nblk)
+    double hr[OT][4], hi[OT][4];
 #pragma unroll
     for (int o = 0; o < OT; ++o) {
-      xr[o] = er[o];
-      xi[o] = ei[o];
-    }
-    const int n_ms = (n_t - k0 + 3) >> 2;
-    // this lane's weight row of matrix step 0 (row r <-> step kbase + r); the
-    // rows of a block of four matrix steps sit at immediate offsets from it
-    const double* q =
-        vt + (static_cast<int64_t>(i) * info.len + (k0 - info.kbase) + kk) * NCOL;
-    // B operands of four matrix steps, the next four loaded while these run.
-    // Loads run up to 28 rows past a ring's last row: into the next ring's
-    // rows or the buffer's padding (kMatrixRowPad), never used.
-    auto load4 = [&](const double* qb, double (&b)[4][CT]) {
+      const double tr = __builtin_fma(lr[o], a0r[o], li[o] * a0i[o]);  // A_last,prev conj(A_0)
+      const double tim = __builtin_fma(li[o], a0r[o], -(lr[o] * a0i[o]));
+      const double rr = __builtin_fma(tr, ebr[o], -(tim * ebi[o]));
+      const double rim = __builtin_fma(tr, ebi[o], tim * ebr[o]);
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int r = 0; r < 4; ++r) {
+        const int src = kk + 4 * r;
+        hr[o][r] = __shfl(ebr[o], src);
+        hi[o][r] = __shfl(ebi[o], src);
+        if (any_ring) {
+          const double xr = __shfl(rr, src), xi = __shfl(rim, src);
 #pragma unroll
-        for (int c = 0; c < CT; ++c) b[u][c] = qb[u * 4 * NCOL + c * 16];
-    };
-    auto run4 = [&](int ms0, const double (&b)[4][CT]) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int ms = ms0 + u;
-        if (ms >= n_ms) break;
-        const int kb = k0 + 4 * ms;
-        if (ms % kMxAnchorSteps == 0 && ms > 0) {
-          // next anchor = anchor * e^{i w 4 kMxAnchorSteps dt (1 - c)}
-#pragma unroll
-          for (int o = 0; o < OT; ++o) {
-            const double nr = __builtin_fma(xr[o], fr[o], -(xi[o] * fi[o]));
-            const double nim = __builtin_fma(xr[o], fi[o], xi[o] * fr[o]);
-            xr[o] = nr;
-            xi[o] = nim;
-            er[o] = nr;
-            ei[o] = nim;
+          for (int c = 0; c < CT; ++c) {
+            const double yr = acc_re[o][c][r], yi = acc_im[o][c][r];
+            acc_re[o][c][r] = __builtin_fma(yr, xr, -(yi * xi));
+            acc_im[o][c][r] = __builtin_fma(yr, xi, yi * xr);
           }
         }
+      }
+      lr[o] = alr[o];
+      li[o] = ali[o];
+    }
+    any_ring = true;
+    // this lane's column of block 0 (row r <-> step kbase + r)
+    const double* q = vt + (static_cast<int64_t>(i) * info.len + (k0 - info.kbase)) * NCOL;
+    // B operands of one block (the pair rows a and b of every slice), the next
+    // block's loaded while this one runs.  The loads of the block after a
+    // ring's last run up to kMxBlock rows past it: into the next ring's rows or
+    // the buffer's padding (kMatrixRowPad), never used.
+    auto loadb = [&](const double* qb, double (&va)[NSL][CT], double (&vb)[NSL][CT]) {
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl)
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-          if (kb + 4 <= ct_lo[c]) continue;  // the tile's columns have no weight yet
+          va[sl][c] = qb[off_a[sl] + c * 16];
+          vb[sl][c] = qb[off_b[sl] + c * 16];
+        }
+    };
+    auto runb = [&](int blk, const double (&va)[NSL][CT], const double (&vb)[NSL][CT]) {
+      const int kb = k0 + kMxBlock * blk;
 #pragma unroll
-          for (int o = 0; o < OT; ++o) {
-            acc_re[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[o], b[u][c], acc_re[o][c], 0, 0, 0);
-            acc_im[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[o], b[u][c], acc_im[o][c], 0, 0, 0);
+      for (int c = 0; c < CT; ++c) {
+        // H <- H conj(E_B), every block of the ring
+#pragma unroll
+        for (int o = 0; o < OT; ++o)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double yr = acc_re[o][c][r], yi = acc_im[o][c][r];
+            acc_re[o][c][r] = __builtin_fma(yr, hr[o][r], yi * hi[o][r]);
+            acc_im[o][c][r] = __builtin_fma(yi, hr[o][r], -(yr * hi[o][r]));
           }
+        // then the block sum about its centre, unless the column tile has no
+        // weight in the block (its rows there are not written; adding selected
+        // zeros instead measured slower: 11.6 against 11.4 ms at plain_256)
+        if (kb + kMxBlock <= ct_lo[c] || ct_lo[c] >= n_t) continue;
+        double sv[NSL], dv[NSL];
+#pragma unroll
+        for (int sl = 0; sl < NSL; ++sl) {
+          sv[sl] = va[sl][c] + vb[sl][c];
+          dv[sl] = va[sl][c] - vb[sl][c];
         }
 #pragma unroll
-        for (int o = 0; o < OT; ++o) {
-          const double nr = __builtin_fma(er[o], sr[o], -(ei[o] * si[o]));
-          const double nim = __builtin_fma(er[o], si[o], ei[o] * sr[o]);
-          er[o] = nr;
-          ei[o] = nim;
+        for (int sl = 0; sl < NSL; ++sl) {
+#pragma unroll
+          for (int o = 0; o < OT; ++o) {
+            acc_re[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(far[o][sl], sv[sl], acc_re[o][c], 0, 0, 0);
+            acc_im[o][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fai[o][sl], dv[sl], acc_im[o][c], 0, 0, 0);
+          }
         }
       }
     };
-    double ba[4][CT], bb[4][CT];
-    load4(q, ba);
-    for (int ms0 = 0; ms0 < n_ms; ms0 += 8) {
-      load4(q + static_cast<int64_t>(ms0 + 4) * 4 * NCOL, bb);
-      run4(ms0, ba);
-      if (ms0 + 4 >= n_ms) break;
-      load4(q + static_cast<int64_t>(ms0 + 8) * 4 * NCOL, ba);
-      run4(ms0 + 4, bb);
+    double xa[NSL][CT], xb[NSL][CT], ya[NSL][CT], yb[NSL][CT];
+    loadb(q, xa, xb);
+    for (int blk = 0; blk < nblk; blk += 2) {
+      loadb(q + static_cast<int64_t>(blk + 1) * kMxBlock * NCOL, ya, yb);
+      runb(blk, xa, xb);
+      if (blk + 1 >= nblk) break;
+      loadb(q + static_cast<int64_t>(blk + 2) * kMxBlock * NCOL, xa, xb);
+      runb(blk + 1, ya, yb);
     }
+  }
+  // the sum of the rings: A_last H at the element rows
+  if (any_ring) {
+#pragma unroll
+    for (int o = 0; o < OT; ++o)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int src = kk + 4 * r;
+        const double xr = __shfl(lr[o], src), xi = __shfl(li[o], src);
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          const double yr = acc_re[o][c][r], yi = acc_im[o][c][r];
+          acc_re[o][c][r] = __builtin_fma(yr, xr, -(yi * xi));
+          acc_im[o][c][r] = __builtin_fma(yr, xi, yi * xr);
+        }
+      }
   }
 
   // C_item(w) = exp(-i w z_a) Y[w, col]: element r of lane L holds row
