@@ -8,10 +8,11 @@ grid; 1000 time steps to full percolation; 256 omegas geomspace(0.5, 500);
 S(k-hat, omega) over 64 x 256 points: first-collision geometry, phase sums,
 per-k-hat reduction.
 
-Three forms run on the same inputs: the plain FP64 form (one FP64 sincos per
+Four forms run on the same inputs: the plain FP64 form (one FP64 sincos per
 term), the blocked FP64 form (phases factorised over blocks of time steps,
-FP64 throughout) and the mixed form (FP64 masks and argument reduction, FP32
-sincos and Kahan-compensated FP32 accumulation).  Each form's per-bin error
+FP64 throughout), the mixed form (FP64 masks, argument reduction, anchors and
+accumulation, FP32 phase tables, weights and block sums) and the FP64 matrix
+form (the phase sums on the FP64 matrix cores).  Each form's per-bin error
 against the plain FP64 form is reported.  `value` is the (omega, k-hat)
 points/s of the fastest form whose largest per-bin relative error is
 <= 1e-6, from the device time of the step (HIP events on the library's
@@ -511,7 +512,7 @@ def main():
         return None if e is None or not ms else e["total"] / (ms * 1e-3) / 1e12
 
     dtypes = {"fp64": "f64", "fp64_blocked": "f64", "fp64_matrix": "f64",
-              "mixed": "f32 sincos/accumulation, f64 masks and argument reduction"}
+              "mixed": "f32 phase tables, weights and block sums; f64 masks, argument reduction, anchors and accumulation"}
     tr = (traffic.get(main_mode) or {})
     if main_mode == "fp64_matrix":
         # the dominant kernel: the matrix-core products (HIP events around its launches)
@@ -519,7 +520,9 @@ def main():
         basis = ("executed FP64 flops per launch: matrix cores (rocprofv3 "
                  "SQ_INSTS_VALU_MFMA_MOPS_F64 x 512) + vector (SQ_INSTS_VALU_FLOPS_FP64 x 64 "
                  "lanes), same workload, / the kernel's HIP-event time; peak: MI355X dense "
-                 "FP64 matrix")
+                 "FP64 matrix.  The products pair conjugate steps about 16-step block "
+                 "centres: half the matrix flops of the direct phase sum for the same "
+                 "outputs, plus the vector Horner steps that combine the blocks")
     else:
         kern, part, ms, bound = f"spectrum_kernel ({main_mode} form)", "spectrum", r["spectrum_ms"], "valu"
         basis = ("executed FP64 VALU flops per launch (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 "
