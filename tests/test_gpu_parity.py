@@ -469,6 +469,24 @@ def test_reference_example_configuration_conf_y():
     assert rel_per_bin(S, So) <= FP64_REL
 
 
+@pytest.mark.parametrize("n_t", [5, 16, 17, 47, 64])
+@pytest.mark.parametrize("sets", [([[1.0]], [[5.0]]), ([[0.7, 0.3], [1.0, 0.1]], [[5.0, 3.5], [4.0, 6.0]])])
+def test_matrix_form_block_edges(n_t, sets):
+    """The matrix form's products run in blocks of 16 steps about their centres on each
+    tile's grid: step counts inside one block, at a block edge and one past it, with the
+    bubbles' first steps (and so their tiles' grids) spread over the time range; against
+    the oracle per bin."""
+    coef, pows = sets
+    b, ext = small_async(23, n=7)
+    k = mb.fibonacci_directions(12)
+    w = np.geomspace(0.5, 150.0, 24)
+    bf = flow(b, ext, 12, 24, coef, pows)
+    t_end = float(b[:, 0].max()) + 0.6
+    So = oracle.spectrum(b, ext, k, 12, 24, coef, pows, w, t_end=t_end, n_t=n_t)
+    Sm, _ = bf.compute_spectrum(w, k, t_end=t_end, n_t=n_t, precision="fp64_matrix")
+    assert rel_per_bin(Sm, So) <= FP64_REL
+
+
 @pytest.mark.parametrize("n_k", [5, 12, 40])
 @pytest.mark.parametrize("sets", [([[1.0]], [[5.0]]), ([[0.7, 0.3], [1.0, 0.1]], [[5.0, 3.5], [4.0, 6.0]])])
 @pytest.mark.parametrize("block_steps", [1, 2, 4])
