@@ -72,10 +72,12 @@ typedef struct bgw_generalized_bulk_flow bgw_generalized_bulk_flow;
  *  Fp64Blocked: the same block factorisation entirely in double.
  *  Fp64Matrix:  double throughout, on the matrix cores: the phase sums of the
  *               items of one bubble in several k-hat frames (same t_a) are one
- *               matrix product per ring (phase factors x weights), accumulated
- *               by FP64 MFMA; fast when many k-hat share the call (spectra);
- *               no damping width; n_t <= 65535; single configurations only
- *               (not the batched ensembles). */
+ *               matrix product per ring (half-step phase factors x the sums
+ *               and differences of the weights paired about the centres of
+ *               16-step blocks), accumulated by FP64 MFMA; fast when many
+ *               k-hat share the call (spectra); no damping width;
+ *               n_t <= 65535; single configurations and the batched
+ *               ensembles. */
 typedef enum BgwPrecision {
     BgwPrecision_Fp64 = 0,
     BgwPrecision_Mixed = 1,
