@@ -533,14 +533,37 @@ struct Request {
   bool item_order = true;
 };
 
-// Indices [0, n) ordered by cost descending, ties in index order (the order of
-// a stable sort on cost), sorted as contiguous (cost, index) pairs.
+// The indices idx[0, n) (given in ascending order) reordered by cost
+// descending: a counting sort on the cost quantised to kCostLevels levels
+// between the range's extremes, index order within a level.  The order only
+// balances the launches (every item writes its own partial result, reduced in
+// a fixed order), so levels of 1/4096 of the range serve as well as an exact
+// sort, in linear time (ensemble_512: 14 ms against 175 ms for sorting its
+// 2.09 M items in chunks of 65536).
 template <typename CostAt>
 void order_by_cost(int32_t* idx, size_t n, CostAt cost_at) {
-  std::vector<std::pair<double, int32_t>> key(n);
-  for (size_t i = 0; i < n; ++i) key[i] = {-cost_at(idx[i]), idx[i]};
-  std::sort(key.begin(), key.end());
-  for (size_t i = 0; i < n; ++i) idx[i] = key[i].second;
+  if (n < 2) return;
+  constexpr int kCostLevels = 4096;
+  double lo = cost_at(idx[0]), hi = lo;
+  for (size_t i = 1; i < n; ++i) {
+    const double c = cost_at(idx[i]);
+    lo = std::min(lo, c);
+    hi = std::max(hi, c);
+  }
+  const double scale = hi > lo ? kCostLevels / (hi - lo) : 0.0;
+  auto level = [&](double c) {
+    const int b = static_cast<int>((hi - c) * scale);  // 0: the most expensive
+    return b < 0 ? 0 : (b >= kCostLevels ? kCostLevels - 1 : b);
+  };
+  std::vector<int32_t> start(kCostLevels + 1, 0), out(n);
+  std::vector<int16_t> lev(n);
+  for (size_t i = 0; i < n; ++i) {
+    lev[i] = static_cast<int16_t>(level(cost_at(idx[i])));
+    ++start[lev[i] + 1];
+  }
+  for (int b = 0; b < kCostLevels; ++b) start[b + 1] += start[b];
+  for (size_t i = 0; i < n; ++i) out[start[lev[i]]++] = idx[i];
+  std::copy(out.begin(), out.end(), idx);
 }
 
 struct Result {
