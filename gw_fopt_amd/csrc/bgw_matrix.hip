@@ -512,8 +512,7 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
     li[o] = 0.0;
   }
   bool any_ring = false;
-  // the element rows of this lane's accumulators: omega row kk + 4 r of each
-  // omega tile, held (as the A operand's row) by lane kk + 4 r
+  // this lane's column of every weight row: lane % 16 of each column tile
   const double* vt = m.vbuf + (info.voff - m.voff0) + (lane & 15);
   // this lane's rows of a block (kMxBlock steps from its start), for the
   // pairs j = 4 sl + kk about the block's centre: a = H + j, b = H - 1 - j,
@@ -581,9 +580,11 @@ __global__ __launch_bounds__(256) void mx_gemm_kernel(MatArgs m) {
       sincos_f64(om[o] * (tc0 * ai + t_a * ci), &a0i[o], &a0r[o]);
       sincos_f64(om[o] * (tcl * ai + t_a * ci), &ali[o], &alr[o]);
     }
-    // the Horner factor conj(E_B) at the element rows, and the carry of the
-    // processed rings into this ring's H: H <- H A_last,prev conj(A_0) E_B
-    // (the ring's blocks then multiply it by conj(E_B)^nblk)
+    // the Horner factor conj(E_B) at the element rows (element r of this lane's
+    // accumulators is omega row kk + 4 r of its omega tile, whose factors lane
+    // kk + 4 r holds for the A operand), and the carry of the processed rings
+    // into this ring's H: H <- H A_last,prev conj(A_0) E_B (the ring's blocks
+    // then multiply it by conj(E_B)^nblk)
     double hr[OT][4], hi[OT][4];
 #pragma unroll
     for (int o = 0; o < OT; ++o) {
