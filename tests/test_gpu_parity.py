@@ -273,6 +273,29 @@ def test_blocked_and_mixed_block_lengths(block_steps, n_t):
     assert np.median(rel) <= MIXED_REL
 
 
+@pytest.mark.parametrize("block_steps", [8, 16])
+def test_blocked_and_mixed_three_sets(block_steps):
+    """Three sets of two terms in one launch (the pair sums and differences of every
+    set's weights about each block's centre): blocked against the oracle per bin, mixed
+    against FP64, on a step count that ends in a partial block."""
+    inter, ext = small_async(12, n=7)
+    coef = [[0.0, 0.0], [1.0, 0.5], [0.3, 0.7]]
+    pows = [[3.0, 4.0], [5.0, 3.5], [4.2, 6.0]]
+    w = np.geomspace(0.5, 120.0, 33)
+    k = mb.fibonacci_directions(5)
+    bf = flow(inter, ext, 16, 24, coef, pows)
+    t_end = float(inter[:, 0].max()) + 0.6
+    So = oracle.spectrum(inter, ext, k, 16, 24, coef, pows, w, t_end=t_end, n_t=93)
+    bf.set_tiles(block_steps, 0)
+    Sb, _ = bf.compute_spectrum(w, k, t_end=t_end, n_t=93, precision="fp64_blocked")
+    assert rel_per_bin(Sb, So) <= FP64_REL
+    S64, _ = bf.compute_spectrum(w, k, t_end=t_end, n_t=93)
+    S32, _ = bf.compute_spectrum(w, k, t_end=t_end, n_t=93, precision="mixed")
+    rel = np.abs(S32 - S64) / np.abs(S64)
+    assert rel.max() <= 1e-4
+    assert np.median(rel) <= MIXED_REL
+
+
 def test_identity_frame_equals_reference_frame():
     """k-hat = z-hat reproduces the reference-frame call bit for bit."""
     wl = configs.plain_8()
